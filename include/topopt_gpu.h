@@ -1,0 +1,162 @@
+/*
+ * C-ABI of the MI355X (gfx950) transient-elasticity topology-optimisation path.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Every function returns 0 on
+ * success and a negative code on failure; tt_last_error() gives the message.
+ * Krylov non-convergence and non-finite values are reported as errors
+ * (TT_ERR_NOT_CONVERGED / TT_ERR_NONFINITE), not silently accepted.
+ *
+ * Pointer arguments marked "array" are host pointers when on_device == 0
+ * (copied in/out synchronously) and device pointers on the context's device
+ * when on_device == 1 (used in place; the caller keeps ownership).
+ *
+ * Each entry point names the reference interface it replaces
+ * (file:line in topopt/TopOpt_in_PETSc_Transient).
+ */
+#ifndef TOPOPT_GPU_H
+#define TOPOPT_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TT_OK 0
+#define TT_ERR_ARG (-1)
+#define TT_ERR_HIP (-2)
+#define TT_ERR_INTERNAL (-3)
+#define TT_ERR_NOT_CONVERGED (-4)
+#define TT_ERR_NONFINITE (-5)
+#define TT_ERR_STATE (-6)
+
+#define TT_PREC_FP64 0  /* FP64 flexible CG, FP64 multigrid */
+#define TT_PREC_MIXED 1 /* FP64 flexible CG, FP32 multigrid, BF16 coarse coefficients, FP64 coarsest solve */
+
+#define TT_OP_K 0 /* stiffness K(rho) */
+#define TT_OP_M 1 /* mass M(rho) */
+#define TT_OP_A 2 /* Newmark system K + a2 C + a5 M = (1 + a2 dbeta) K + (a5 + a2 dalpha) M */
+
+typedef struct tt_ctx tt_ctx;
+typedef struct tt_mma tt_mma;
+
+/* Problem, material, time-integration and solver parameters.  Names follow
+   the reference options (TopOpt.cc:71-165, 329-518, LinearElasticity.h:29). */
+typedef struct tt_params {
+    int nelx, nely, nelz;          /* elements; nodes are nel+1 (TopOpt.cc:179-181 -nx/-ny/-nz minus one) */
+    double xc[6];                  /* domain box xmin,xmax,ymin,ymax,zmin,zmax (TopOpt.cc:185-190) */
+    double nu;                     /* Poisson ratio (-nu) */
+    int sInt, mInt;                /* stiffness / mass interpolation 0 linear, 1 SIMP, 2 RAMP */
+    double Emin, Emax, Rmin, Rmax; /* Emin = Ef*Emax, Rmin = Rf*Rmax (TopOpt.cc:410-413) */
+    double penalK, penalM;
+    double dalpha, dbeta;          /* Rayleigh damping C = dalpha M + dbeta K (-dalph, -dbeta) */
+    int ti;                        /* 0 backward Euler, 1 Newmark beta=1/4 gamma=1/2 (LinearElasticity.cc:185-207) */
+    int nsteps;                    /* time steps after t = 0: Ts - 1 = ncp*tsc */
+    double t1;                     /* final time; dt = t1/nsteps */
+    int objective;                 /* 0..6 (LinearElasticity.cc:1416-1465) */
+    int boundaries;                /* 0 MBB, 1 cantilever (LinearElasticity.cc:550-575) */
+    int loadloc;                   /* 0,1 reference presets; 2 y-line load at x = xmax, z = zmin */
+    int loadsignal;                /* 0 reference Gaussian-cosine pulse; 1 half-sine over pulse_steps */
+    int pulse_steps;
+    int region;                    /* optimizationRegion 0..3 (LinearElasticity.cc:222-335) */
+    int nlvls;                     /* multigrid levels (-nlvls) */
+    int precision;                 /* TT_PREC_FP64 or TT_PREC_MIXED */
+    double rtol;                   /* relative tolerance of ||b - A x|| / ||b|| */
+    int maxit;                     /* flexible-CG iteration cap per solve */
+    int cheb_degree;               /* Chebyshev smoother degree on D^-1 A */
+    int checkpoint_slots;          /* binomial checkpoint slots; >= nsteps stores every state */
+    double rmin;                   /* PDE filter radius (-rmin); R = rmin/(2 sqrt 3) */
+    double filter_rtol;            /* PDE filter solve tolerance */
+    int verbose;
+} tt_params;
+
+typedef struct tt_stats {
+    long solves;          /* linear solves since the last reset */
+    long krylov_iters;    /* flexible-CG iterations since the last reset */
+    int last_iters;
+    double last_relres;   /* true relative residual of the last solve */
+    double max_relres;
+    long forward_steps;   /* Newmark steps integrated (incl. recomputation) */
+    long adjoint_steps;
+    double lambda_max[8]; /* Chebyshev upper eigenvalue estimate per level */
+    int nlvls;
+} tt_stats;
+
+/* Defaults of the cantilever pulse problem (3-level MG, FP64, rtol 1e-8). */
+int tt_default_params(tt_params* prm);
+
+/* Replaces the construction of LinearElasticity (LinearElasticity.cc:14-120, called at
+   main.cc:35): meshes, BC mask (SetUpBC :337-591), load location (:616-675), region
+   (:222-335), Newmark constants (:185-219), element matrices (:516-519). */
+int tt_create(const tt_params* prm, tt_ctx** ctx);
+void tt_destroy(tt_ctx* ctx);
+const char* tt_last_error(const tt_ctx* ctx); /* ctx may be NULL: last creation error */
+
+/* xPhys input of CompleteInitialState / ComputeObjectiveSensitivities (main.cc:82,90);
+   replaces AssembleStiff/Mass/DampingMatrix (LinearElasticity.cc:2502-2619): element
+   interpolation only, nothing is assembled.  nel doubles. */
+int tt_set_density(tt_ctx* ctx, const double* xphys, int on_device);
+
+/* One matrix-free operator application y = N.(X (N.u)) + (1-N).u for X in {K, M, A}
+   (the MatMult of LinearElasticity.cc:1342-1345); prec 0 = FP64 arrays, 1 = FP32 arrays.
+   3*nn entries. */
+int tt_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int on_device);
+
+/* CompleteInitialState (LinearElasticity.cc:956-1206): multigrid set-up for the Newmark
+   operator and the initial-acceleration solve (N M N + I - N) a0 = N.(f(0) - C v0 - K u0). */
+int tt_complete_initial_state(tt_ctx* ctx);
+
+/* GenerateCheckPoints + SolveStateBlock (LinearElasticity.cc:1208-1280, 1298-1397): the
+   whole Newmark forward sweep, keeping binomial checkpoints (or every state). */
+int tt_forward(tt_ctx* ctx);
+
+/* State at step t (0..nsteps) when every state is stored; 3*nn doubles each (may be NULL). */
+int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device);
+
+/* ComputeObjectiveSensitivities (LinearElasticity.cc:1625-1908): objective fx and dfdx
+   (nel) by the backward Newmark adjoint (SolveLagrangen :808-954, SolveLagrange0
+   :678-804) with recomputation between checkpoints. */
+int tt_objective_gradient(tt_ctx* ctx, double* fx, double* dfdx, int on_device);
+
+/* VolumeConstraint (LinearElasticity.cc:1283-1296): gx = sum(xPhys)/nel - volfrac, dgdx = 1/nel. */
+int tt_volume_constraint(tt_ctx* ctx, const double* xphys, double volfrac, double* gx, double* dgdx,
+                         int on_device);
+
+/* PDEFilt::FilterProject / Gradients (PDEFilter.cc:193-224): out = T' K_F^-1 (V T in). */
+int tt_pde_filter(tt_ctx* ctx, const double* in, double* out, int on_device);
+
+/* Filter::FilterProject with filter = 2 (Filter.cc:60-119): PDE filter, clamp to [0,1],
+   optional Heaviside projection (Filter.h:80-83). */
+int tt_filter_project(tt_ctx* ctx, const double* x, double* xtilde, double* xphys, int projection, double beta,
+                      double eta, int on_device);
+
+/* Filter::Gradients with filter = 2, m = 1 (Filter.cc:122-206), in place on dfdx and dgdx. */
+int tt_filter_gradients(tt_ctx* ctx, const double* xtilde, double* dfdx, double* dgdx, int projection,
+                        double beta, double eta, int on_device);
+
+/* Filter::GetMND (Filter.cc:208-227). */
+int tt_mnd(tt_ctx* ctx, const double* x, double* mnd, int on_device);
+
+/* MMA with m = 1 constraint (MMA.h:12-53, MMA.cc).  a, c, d as in TopOpt.cc:527-531. */
+int tt_mma_create(tt_ctx* ctx, double a, double c, double d, tt_mma** mma);
+void tt_mma_destroy(tt_mma* mma);
+int tt_mma_set_asymptotes(tt_mma* mma, double init, double decrease, double increase); /* MMA.cc:352-360 */
+int tt_mma_set_outer_movelimit(tt_mma* mma, double Xmin, double Xmax, double movlim, const double* x,
+                               double* xmin, double* xmax, int on_device);             /* MMA.cc:374-393 */
+int tt_mma_update(tt_mma* mma, double* x, const double* dfdx, const double* gx, const double* dgdx,
+                  const double* xmin, const double* xmax, int on_device);               /* MMA.cc:487-510 */
+int tt_mma_design_change(tt_mma* mma, const double* x, double* xold, double* ch, int on_device); /* :395-414 */
+int tt_mma_kkt_residual(tt_mma* mma, const double* x, const double* dfdx, const double* gx, const double* dgdx,
+                        const double* xmin, const double* xmax, double* norm2, double* normInf,
+                        int on_device);                                                  /* MMA.cc:416-484 */
+int tt_mma_get_asymptotes(tt_mma* mma, double* L, double* U, double* lam); /* host arrays (nel, nel, 1) */
+
+int tt_get_stats(tt_ctx* ctx, tt_stats* st);
+int tt_reset_stats(tt_ctx* ctx);
+int tt_synchronize(tt_ctx* ctx);
+/* Sizes: nodes, elements. */
+long tt_num_nodes(const tt_ctx* ctx);
+long tt_num_elements(const tt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,189 @@
+"""HIP path against the FP64 CPU oracle, through the C-ABI (small grids).
+
+Bars: FP64 path within 1e-8 relative of the oracle, mixed-precision path
+within 1e-5 relative; integer indexing (BC mask, load location) exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_helpers import problem, random_density, rel
+from topopt_in_petsc_transient_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-8
+MIXED_TOL = 1e-5
+
+
+def _ctx(pg, x):
+    c = capi.Context(pg)
+    c.set_density(x)
+    return c
+
+
+@pytest.mark.parametrize("which,cK,cM", [(capi.TT_OP_K, 1.0, 0.0), (capi.TT_OP_M, 0.0, 1.0), (capi.TT_OP_A, None, None)])
+def test_operator_apply_fp64_and_fp32(which, cK, cM):
+    po, pg = problem(nelx=16, nely=8, nelz=8)
+    x = random_density(O.nel(po), seed=1)
+    c = _ctx(pg, x)
+    if cK is None:
+        a = O.newmark_consts(1, po.t1 / po.nsteps)
+        cK, cM = 1.0 + a[2] * po.dbeta, a[5] + a[2] * po.dalpha
+    N = O.bc_mask(po)
+    u = np.random.default_rng(2).standard_normal(c.nd) * N
+    ref = O.apply_bc(po, x, cK, cM, u, N)
+    y = np.zeros(c.nd)
+    c.op_apply(which, u, y, prec=0)
+    assert rel(y, ref) < 1e-13
+    y32 = np.zeros(c.nd, dtype=np.float32)
+    c.op_apply(which, u.astype(np.float32), y32, prec=1)
+    assert rel(y32, ref) < 1e-5
+
+
+def test_forward_states_fp64():
+    po, pg = problem()
+    x = random_density(O.nel(po), seed=3)
+    states = O.forward(po, x)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    for t in (0, 1, 5, 10, 29, 30):
+        u, v, a = c.get_state(t)
+        for q, arr in enumerate((u, v, a)):
+            if np.linalg.norm(states[t, q]) == 0:
+                assert np.linalg.norm(arr) == 0
+            else:
+                assert rel(arr, states[t, q]) < FP64_TOL, (t, q)
+
+
+def test_objective_gradient_fp64():
+    po, pg = problem()
+    x = random_density(O.nel(po), seed=4)
+    states = O.forward(po, x)
+    fx_ref, df_ref = O.objective_gradient(po, x, states)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    fx, df = c.objective_gradient()
+    assert abs(fx - fx_ref) / abs(fx_ref) < FP64_TOL
+    assert rel(df, df_ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("objective", [0, 1, 2, 4, 5, 6])
+def test_other_objectives_fp64(objective):
+    po, pg = problem(nelx=8, nely=4, nelz=4, nsteps=12, dt=0.02, pulse_steps=5, objective=objective, nlvls=2)
+    x = random_density(O.nel(po), seed=5)
+    states = O.forward(po, x)
+    fx_ref, df_ref = O.objective_gradient(po, x, states)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    fx, df = c.objective_gradient()
+    assert abs(fx - fx_ref) / abs(fx_ref) < FP64_TOL
+    assert rel(df, df_ref) < FP64_TOL
+
+
+def test_revolve_matches_store_all_bitwise():
+    _, pg = problem(nsteps=30)
+    x = random_density(pg.nelx * pg.nely * pg.nelz, seed=6)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    fx0, df0 = c.objective_gradient()
+    _, pg2 = problem(nsteps=30, slots=4)
+    c2 = _ctx(pg2, x)
+    c2.complete_initial_state()
+    c2.forward()
+    fx1, df1 = c2.objective_gradient()
+    s = c2.stats()
+    assert fx1 == fx0
+    assert np.array_equal(df1, df0)
+    assert s.adjoint_steps == 30 and s.forward_steps > 30
+
+
+def test_mixed_precision_against_oracle():
+    po, pg = problem(precision=capi.TT_PREC_MIXED, rtol=1e-10)
+    x = random_density(O.nel(po), seed=7)
+    states = O.forward(po, x)
+    fx_ref, df_ref = O.objective_gradient(po, x, states)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    u, _, _ = c.get_state(30)
+    assert rel(u, states[30, 0]) < MIXED_TOL
+    fx, df = c.objective_gradient()
+    assert abs(fx - fx_ref) / abs(fx_ref) < MIXED_TOL
+    assert rel(df, df_ref) < MIXED_TOL
+
+
+def test_pde_filter_and_projection():
+    po, pg = problem()
+    x = random_density(O.nel(po), seed=8, lo=0.0, hi=1.0)
+    c = capi.Context(pg)
+    ref = O.pde_filter(po, 0.08, x)
+    out = c.pde_filter(x)
+    assert rel(out, ref) < 1e-10
+    for proj in (False, True):
+        xt_r, xp_r = O.filter_project(po, 0.08, x, projection=proj, beta=2.0, eta=0.3)
+        xt, xp = c.filter_project(x, projection=proj, beta=2.0, eta=0.3)
+        assert rel(xt, xt_r) < 1e-10 and rel(xp, xp_r) < 1e-10
+        df = np.random.default_rng(9).standard_normal(x.size)
+        dg = np.full(x.size, 1.0 / x.size)
+        a_r, b_r = O.filter_gradients(po, 0.08, xt_r, df, dg, projection=proj, beta=2.0, eta=0.3)
+        a, b = c.filter_gradients(xt, df, dg, projection=proj, beta=2.0, eta=0.3)
+        assert rel(a, a_r) < 1e-10 and rel(b, b_r) < 1e-10
+    assert abs(c.mnd(x) - np.mean(4 * x * (1 - x))) < 1e-14
+
+
+def test_mma_update_sequence_matches_oracle():
+    po, pg = problem(nelx=8, nely=4, nelz=4)
+    n = O.nel(po)
+    c = capi.Context(pg)
+    rng = np.random.default_rng(10)
+    x = np.full(n, 0.3)
+    m_ref = O.MMA(n, 1, x)
+    m_ref.set_asymptotes(0.1, 0.6, 1.05)
+    m = capi.MMA(c)
+    m.set_asymptotes(0.1, 0.6, 1.05)
+    xr = x.copy()
+    xg = x.copy()
+    xold_r = x.copy()
+    xold_g = x.copy()
+    for it in range(5):
+        df = rng.standard_normal(n) - 0.5
+        dg = np.full(n, 1.0 / n)
+        gx = np.mean(xr) - 0.25
+        lo_r, hi_r = O.MMA.outer_movelimit(0.0, 1.0, 0.2, xr)
+        lo, hi = m.outer_movelimit(0.0, 1.0, 0.2, xg)
+        lo_s, hi_s = O.MMA.outer_movelimit(0.0, 1.0, 0.2, xg)  # same input: element-wise ops are exact
+        assert np.array_equal(lo, lo_s) and np.array_equal(hi, hi_s)
+        xr = m_ref.update(xr, df, gx, dg, lo_r, hi_r)
+        xg = m.update(xg, df, gx, dg, lo, hi)
+        assert rel(xg, xr) < 1e-10, it
+        ch_r, xold_r = O.MMA.design_change(xr, xold_r)
+        ch, xold_g = m.design_change(xg, xold_g)
+        assert abs(ch - ch_r) < 1e-10
+        k_r = m_ref.kkt_residual(xr, df, gx, dg, lo_r, hi_r)
+        k = m.kkt_residual(xg, df, gx, dg, lo, hi)
+        assert abs(k[0] - k_r[0]) <= 1e-8 * max(1.0, abs(k_r[0]))
+
+
+def test_nonconvergence_is_an_error():
+    po, pg = problem(nelx=8, nely=4, nelz=4, nsteps=3, maxit=1, rtol=1e-14)
+    x = random_density(O.nel(po), seed=11)
+    c = _ctx(pg, x)
+    with pytest.raises(capi.TTError) as e:
+        c.complete_initial_state()
+        c.forward()
+    assert e.value.code == capi.TT_ERR_NOT_CONVERGED
+
+
+def test_nonfinite_density_is_an_error():
+    _, pg = problem(nelx=8, nely=4, nelz=4, nsteps=3)
+    c = capi.Context(pg)
+    x = np.full(pg.nelx * pg.nely * pg.nelz, 0.5)
+    x[3] = np.nan
+    with pytest.raises(capi.TTError) as e:
+        c.set_density(x)
+    assert e.value.code == capi.TT_ERR_NONFINITE

@@ -1,0 +1,685 @@
+// Context set-up, PDE filter, MMA driver and the extern "C" entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "tt_context.h"
+
+namespace tt {
+
+static void check_params(const tt_params& p) {
+    TT_CHECK(p.nelx > 0 && p.nely > 0 && p.nelz > 0, TT_ERR_ARG, "element counts must be positive");
+    TT_CHECK(p.xc[1] > p.xc[0] && p.xc[3] > p.xc[2] && p.xc[5] > p.xc[4], TT_ERR_ARG, "empty domain box");
+    TT_CHECK(p.nsteps >= 1 && p.t1 > 0.0, TT_ERR_ARG, "need nsteps >= 1 and t1 > 0");
+    TT_CHECK(p.ti == 0 || p.ti == 1, TT_ERR_ARG, "ti must be 0 or 1");
+    TT_CHECK(p.objective >= 0 && p.objective <= 6, TT_ERR_ARG, "objective must be 0..6");
+    TT_CHECK(p.precision == TT_PREC_FP64 || p.precision == TT_PREC_MIXED, TT_ERR_ARG, "bad precision mode");
+    TT_CHECK(p.cheb_degree >= 1 && p.cheb_degree <= 16, TT_ERR_ARG, "cheb_degree must be 1..16");
+    TT_CHECK(p.rtol > 0.0 && p.maxit > 0, TT_ERR_ARG, "need rtol > 0 and maxit > 0");
+    TT_CHECK(p.loadsignal != 1 || p.pulse_steps > 0, TT_ERR_ARG, "pulse_steps must be positive");
+    long nn = (long)(p.nelx + 1) * (p.nely + 1) * (p.nelz + 1);
+    TT_CHECK(3 * nn < (1L << 31), TT_ERR_ARG, "too many dofs for 32-bit indexing");
+}
+
+Context::Context(const tt_params& p) : prm(p) {
+    check_params(p);
+    TT_HIP(hipGetDevice(&device));
+    TT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    g = make_grid(p.nelx, p.nely, p.nelz, p.xc[1] - p.xc[0], p.xc[3] - p.xc[2], p.xc[5] - p.xc[4]);
+    const long nd = g.nd, nn = g.nn, nel = g.nel;
+    const long apply_blocks = (long)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 3) / 4);
+    red.init((int)std::max<long>({apply_blocks, (nel + 255) / 256, (nn + 255) / 256, 2048}));
+
+    // Newmark constants (LinearElasticity.cc:185-207)
+    dt = p.t1 / p.nsteps;
+    if (p.ti == 0) {
+        a[0] = 0.0; a[1] = 0.0; a[2] = 1.0 / dt; a[3] = 1.0 / dt; a[4] = 0.0; a[5] = 1.0 / (dt * dt);
+    } else {
+        const double b = 0.25, gm = 0.5;
+        a[0] = 1.0 - gm / b;
+        a[1] = (1.0 - gm / (2.0 * b)) * dt;
+        a[2] = gm / (b * dt);
+        a[3] = 1 / (b * dt);
+        a[4] = 1 / (2.0 * b) - 1.0;
+        a[5] = 1.0 / (b * dt * dt);
+    }
+    cK_A = 1.0 + a[2] * p.dbeta;
+    cM_A = a[5] + a[2] * p.dalpha;
+
+    hex8_stiffness(g.hx, g.hy, g.hz, p.nu, KE);
+    hex8_mass(g.hx, g.hy, g.hz, M8);
+    float ke32[576], m832[64];
+    for (int i = 0; i < 576; i++) ke32[i] = (float)KE[i];
+    for (int i = 0; i < 64; i++) m832[i] = (float)M8[i];
+    KE64d.alloc(576); M864d.alloc(64); KE32d.alloc(576); M832d.alloc(64);
+    TT_HIP(hipMemcpy(KE64d.p, KE, sizeof(KE), hipMemcpyHostToDevice));
+    TT_HIP(hipMemcpy(M864d.p, M8, sizeof(M8), hipMemcpyHostToDevice));
+    TT_HIP(hipMemcpy(KE32d.p, ke32, sizeof(ke32), hipMemcpyHostToDevice));
+    TT_HIP(hipMemcpy(M832d.p, m832, sizeof(m832), hipMemcpyHostToDevice));
+
+    // Dirichlet mask, load location, optimisation region (index form of the reference's
+    // coordinate tests with tolerance 0.05 h, LinearElasticity.cc:222-335, 540-575, 616-675)
+    std::vector<uint8_t> m(nn, 0);
+    Fhost.assign(nd, 0.0);
+    for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) {
+                const long n = i + (long)g.nx * (j + (long)g.ny * k);
+                uint8_t b = 0;
+                if (p.boundaries == 1) {
+                    if (i == 0) b = 7;
+                } else {
+                    if (i == 0) b |= 3;
+                    if (i == g.nx - 1 && k == 0) b |= 6;
+                }
+                m[n] = b;
+                const bool yend = (j == 0 || j == g.ny - 1);
+                if (p.loadloc == 0) {
+                    if (i == 0 && k == g.nz - 1) Fhost[3 * n + 2] = yend ? 0.5 : 1.0;
+                } else if (p.loadloc == 1) {
+                    if (i == g.nx - 1 && 2 * k == g.ez) Fhost[3 * n + 2] = yend ? 0.5 : 1.0;
+                } else {
+                    if (i == g.nx - 1 && k == 0) Fhost[3 * n + 1] = yend ? 0.5 : 1.0;
+                }
+            }
+    mask.alloc(nn);
+    TT_HIP(hipMemcpy(mask.p, m.data(), nn, hipMemcpyHostToDevice));
+    F.alloc(nd);
+    TT_HIP(hipMemcpy(F.p, Fhost.data(), nd * 8, hipMemcpyHostToDevice));
+    std::vector<double> Ph(nel);
+    {
+        const double X1 = g.ex * g.hx, Y1 = g.ey * g.hy, Z1 = g.ez * g.hz;
+        for (long e = 0; e < nel; e++) {
+            const long i = e % g.ex, j = (e / g.ex) % g.ey, k = e / ((long)g.ex * g.ey);
+            const double ex = (i + 1) * g.hx - g.hx / 2, ey = (j + 1) * g.hy - g.hy / 2, ez = (k + 1) * g.hz - g.hz / 2;
+            bool in = false;
+            switch (p.region) {
+            case 0: in = ez > Z1 - 1 && ex < 0.0 + 1; break;
+            case 1: in = ez > Z1 / 2.0 - 0.5 && ez < Z1 / 2.0 + 0.5 && ex < X1 - 1; break;
+            case 2: in = true; break;
+            default:
+                in = ex > 0.75 * X1 - 2.5 && ex < 0.75 * X1 + 2.5 && ey > 0.50 * Y1 - 2.5 && ey < 0.50 * Y1 + 2.5 &&
+                     ez > 0.50 * Z1 - 2.5 && ez < 0.50 * Z1 + 2.5;
+                break;
+            }
+            Ph[e] = in ? 1.0 : 0.0;
+        }
+    }
+    for (DBuf<double>* b : {&xphys, &kf, &mf, &dkf, &dmf, &P, &Pkf, &Pmf, &df, &ftmp_e, &ftmp_e2}) b->alloc(nel);
+    kf32.alloc(nel);
+    mf32.alloc(nel);
+    TT_HIP(hipMemcpy(P.p, Ph.data(), nel * 8, hipMemcpyHostToDevice));
+    rcnt.alloc(nn);
+    launch_region_count(g, P.p, rcnt.p, stream);
+
+    op_fine64.g = g;
+    op_fine64.coef_kind = COEF_F64;
+    op_fine64.kf = kf.p;
+    op_fine64.mf = mf.p;
+    op_fine64.KE64 = KE64d.p;
+    op_fine64.M864 = M864d.p;
+    op_fine64.KE32 = KE32d.p;
+    op_fine64.M832 = M832d.p;
+    op_fine64.mask = mask.p;
+
+    mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
+
+    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &dinvM, &lam, &lamn, &lamT, &mu, &nu, &mun, &nun, &hdu,
+                            &hddu, &hdddu, &Kl, &Ml, &rhs, &nu0})
+        b->alloc(nd);
+    TT_HIP(hipMemsetAsync(lamT.p, 0, nd * 8, stream));
+    TT_HIP(hipMemsetAsync(nu0.p, 0, nd * 8, stream));
+    fxs.alloc(p.nsteps + 1);
+
+    // state storage: every state, or s binomial slots + the t = 0 state
+    const int T = p.nsteps;
+    int nslots = (p.checkpoint_slots <= 0 || p.checkpoint_slots >= T) ? T + 1 : p.checkpoint_slots + 1;
+    store_all = (nslots == T + 1);
+    size_t freeb = 0, totalb = 0;
+    TT_HIP(hipMemGetInfo(&freeb, &totalb));
+    const double need = (double)nslots * 3 * nd * 8;
+    TT_CHECK(need < 0.85 * (double)freeb, TT_ERR_ARG,
+             "state storage exceeds device memory: lower checkpoint_slots");
+    slots.resize(nslots);
+    for (auto& s : slots) s.alloc(3 * nd);
+    slot_step.assign(nslots, -1);
+    st_cur.alloc(3 * nd);
+    st_next.alloc(3 * nd);
+
+    // PDE filter (PDEFilter.cc:28-191): R = rmin / (2 sqrt 3), KF on the same node grid
+    fg = g;
+    const double R = p.rmin / 2.0 / std::sqrt(3.0);
+    hex8_helmholtz(g.hx, g.hy, g.hz, R, KF);
+    KFd.alloc(64);
+    TT_HIP(hipMemcpy(KFd.p, KF, sizeof(KF), hipMemcpyHostToDevice));
+    for (DBuf<double>* b : {&fdinv, &fU, &frhs, &fr, &fz, &fp, &fq}) b->alloc(nn);
+    launch_filter_diag(fg, KFd.p, fdinv.p, stream);
+    sync();
+}
+
+Context::~Context() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ------------------------------------------------------------------ PDE filter solve (Jacobi PCG, FP64)
+void Context::pde_filter(const double* in, double* out) {
+    const long nn = fg.nn;
+    const double vol = fg.hx * fg.hy * fg.hz;
+    launch_filter_T(fg, in, 1.0, fU.p, stream);   // warm start U = T x (PDEFilter.cc:206)
+    launch_filter_T(fg, in, vol, frhs.p, stream);  // RHS = elemVol * T x (:204-208)
+    launch_filter_apply(fg, KFd.p, fU.p, fq.p, nullptr, 0, stream);
+    launch_lincomb(1.0, frhs.p, -1.0, fq.p, fr.p, nn, stream);
+    launch_dot(frhs.p, frhs.p, nn, red, SC_TMP1, stream);
+    launch_dot(fr.p, fr.p, nn, red, SC_RR, stream);
+    TT_HIP(hipMemcpyAsync(red.host, red.scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, stream));
+    sync();
+    const double bn = std::sqrt(red.host[SC_TMP1]);
+    if (!std::isfinite(bn)) throw Error{TT_ERR_NONFINITE, "non-finite PDE filter input"};
+    const double tol = prm.filter_rtol;
+    double rel = bn > 0 ? std::sqrt(red.host[SC_RR]) / bn : 0.0;
+    if (bn == 0.0) launch_fill(fU.p, 0.0, nn, true, stream);
+    int it = 0;
+    const int maxit = 20000;
+    while (bn > 0.0 && rel > tol && it < maxit) {
+        launch_jacobi(fdinv.p, fr.p, fz.p, nn, stream);
+        launch_lincomb(1.0, fz.p, 0.0, fz.p, fp.p, nn, stream);
+        launch_dot(fr.p, fz.p, nn, red, SC_GAMMA, stream);
+        double rz = read_scal(SC_GAMMA);
+        bool restart = false;
+        while (it < maxit) {
+            it++;
+            launch_filter_apply(fg, KFd.p, fp.p, fq.p, &red, SC_PQ, stream);
+            const double alpha = rz / read_scal(SC_PQ);
+            launch_lincomb(1.0, fU.p, alpha, fp.p, fU.p, nn, stream);
+            launch_lincomb(1.0, fr.p, -alpha, fq.p, fr.p, nn, stream);
+            launch_dot(fr.p, fr.p, nn, red, SC_RR, stream);
+            rel = std::sqrt(read_scal(SC_RR)) / bn;
+            if (!std::isfinite(rel)) throw Error{TT_ERR_NONFINITE, "non-finite PDE filter residual"};
+            if (rel <= tol) break;
+            launch_jacobi(fdinv.p, fr.p, fz.p, nn, stream);
+            launch_dot(fr.p, fz.p, nn, red, SC_GAMMA, stream);
+            const double rzn = read_scal(SC_GAMMA);
+            launch_lincomb(1.0, fz.p, rzn / rz, fp.p, fp.p, nn, stream);
+            rz = rzn;
+        }
+        // true residual check
+        launch_filter_apply(fg, KFd.p, fU.p, fq.p, nullptr, 0, stream);
+        launch_lincomb(1.0, frhs.p, -1.0, fq.p, fr.p, nn, stream);
+        launch_dot(fr.p, fr.p, nn, red, SC_RR, stream);
+        rel = std::sqrt(read_scal(SC_RR)) / bn;
+        (void)restart;
+    }
+    if (rel > tol) throw Error{TT_ERR_NOT_CONVERGED, "PDE filter solve did not converge"};
+    launch_filter_Tt(fg, fU.p, out, stream);  // T' U (:216)
+}
+
+}  // namespace tt
+
+// ====================================================================== MMA (m = 1)
+struct tt_mma {
+    tt::Context* ctx;
+    long n;
+    double a, c, d;
+    int k = 0;
+    double asyminit = 0.5, asymdec = 0.7, asyminc = 1.2;  // MMA.cc:209-211
+    double lam = 0.0, mu = 0.0, y = 0.0, z = 0.0, b = 0.0;
+    tt::DBuf<double> L, U, alpha, beta, p0, q0, pij, qij, xo1, xo2;
+    tt::MMADev dev() {
+        return tt::MMADev{n, L.p, U.p, alpha.p, beta.p, p0.p, q0.p, pij.p, qij.p, xo1.p, xo2.p};
+    }
+};
+
+namespace tt {
+
+// SolveDIP (MMA.cc:645-682) with m = 1: the dual is scalar, so the Newton system is 1x1.
+static void mma_solve_dip(tt_mma* o, double* x) {
+    Context* c = o->ctx;
+    const long n = o->n;
+    o->lam = o->c / 2.0;
+    o->mu = 1.0;
+    const double tol = 1.0e-9 * std::sqrt(1.0 + (double)n);
+    double epsi = 1.0, err = 1.0;
+    auto xyz = [&]() {  // XYZofLAMBDA scalar part (MMA.cc:690-701)
+        if (o->lam < 0.0) o->lam = 0;
+        o->y = std::max(0.0, o->lam - o->c);
+        const double lamai = o->lam * o->a;
+        o->z = std::max(0.0, 10.0 * (lamai - 1.0));
+    };
+    while (epsi > tol) {
+        int loop = 0;
+        while (err > 0.9 * epsi && loop < 100) {
+            loop++;
+            xyz();
+            launch_mma_xyz_grad_hess(o->dev(), o->lam, x, c->red, c->stream);
+            TT_HIP(hipMemcpyAsync(c->red.host + SC_TMP0, c->red.scal.p + SC_TMP0, 2 * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+            c->sync();
+            double grad = c->red.host[SC_TMP0] + (-o->b - o->a * o->z - o->y);  // DualGrad (:729-765)
+            grad = -1.0 * grad - epsi / o->lam;
+            double H = c->red.host[SC_TMP0 + 1];  // DualHess (:767-871)
+            if (o->lam < 0.0) o->lam = 0.0;
+            const double lamai = o->lam * o->a;
+            if (o->lam > o->c) H += -1.0;
+            H += -o->mu / o->lam;
+            if (lamai > 0.0) H += -10.0 * o->a * o->a;
+            double corr = 1e-4 * H / 1.0;
+            if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+            H += corr;
+            const double s0 = grad / H;  // Factorize + Solve of a 1x1 system (:936-969)
+            const double s1 = -o->mu + epsi / o->lam - s0 * o->mu / o->lam;
+            double theta = 1.005;  // DualLineSearch (:873-891)
+            if (theta < -1.01 * s0 / o->lam) theta = -1.01 * s0 / o->lam;
+            if (theta < -1.01 * s1 / o->mu) theta = -1.01 * s1 / o->mu;
+            theta = 1.0 / theta;
+            o->lam = o->lam + theta * s0;
+            o->mu = o->mu + theta * s1;
+            xyz();
+            launch_mma_xyz_res(o->dev(), o->lam, x, c->red, c->stream);
+            const double rs = c->read_scal(SC_TMP0);
+            const double r0 = rs + (-o->b - o->a * o->z - o->y + o->mu);  // DualResidual (:893-934)
+            const double r1 = 0.0 + o->mu * o->lam - epsi;
+            err = std::max(std::fabs(r0), std::fabs(r1));
+            if (!std::isfinite(err)) throw Error{TT_ERR_NONFINITE, "non-finite MMA dual residual"};
+        }
+        epsi = epsi * 0.1;
+    }
+}
+
+}  // namespace tt
+
+// ====================================================================== C-ABI
+using tt::Context;
+using tt::DBuf;
+using tt::Error;
+
+struct tt_ctx {
+    std::unique_ptr<Context> c;
+};
+
+static thread_local std::string g_create_error;
+
+#define TT_API_BEGIN(ctxp)                                    \
+    if (!(ctxp) || !(ctxp)->c) return TT_ERR_ARG;             \
+    Context& C = *(ctxp)->c;                                  \
+    try {
+#define TT_API_END                                            \
+    }                                                         \
+    catch (const Error& e) {                                  \
+        C.last_error = e.msg;                                 \
+        return e.code;                                        \
+    }                                                         \
+    catch (const std::exception& e) {                         \
+        C.last_error = e.what();                              \
+        return TT_ERR_INTERNAL;                               \
+    }                                                         \
+    return TT_OK;
+
+namespace {
+// device view of a caller array: host arrays are staged through a temporary device buffer
+struct DevIn {
+    DBuf<double> tmp;
+    const double* p = nullptr;
+    DevIn(Context& C, const double* src, long n, int on_device) {
+        if (!src) return;
+        if (on_device) {
+            p = src;
+        } else {
+            tmp.alloc(n);
+            TT_HIP(hipMemcpyAsync(tmp.p, src, n * 8, hipMemcpyHostToDevice, C.stream));
+            p = tmp.p;
+        }
+    }
+};
+struct DevOut {
+    Context& C;
+    DBuf<double> tmp;
+    double* p = nullptr;
+    double* host = nullptr;
+    long n;
+    DevOut(Context& C_, double* dst, long n_, int on_device, bool load = false) : C(C_), n(n_) {
+        if (!dst) return;
+        if (on_device) {
+            p = dst;
+        } else {
+            tmp.alloc(n);
+            host = dst;
+            p = tmp.p;
+            if (load) TT_HIP(hipMemcpyAsync(tmp.p, dst, n * 8, hipMemcpyHostToDevice, C.stream));
+        }
+    }
+    void finish() {
+        if (host) TT_HIP(hipMemcpyAsync(host, tmp.p, n * 8, hipMemcpyDeviceToHost, C.stream));
+        TT_HIP(hipStreamSynchronize(C.stream));
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int tt_default_params(tt_params* p) {
+    if (!p) return TT_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->nelx = 32; p->nely = 16; p->nelz = 16;
+    p->xc[0] = 0.0; p->xc[1] = 2.0; p->xc[2] = 0.0; p->xc[3] = 1.0; p->xc[4] = 0.0; p->xc[5] = 1.0;
+    p->nu = 0.3;
+    p->sInt = 1; p->mInt = 0;
+    p->Emax = 1.0; p->Emin = 1e-9; p->Rmax = 1.0; p->Rmin = 1e-3;
+    p->penalK = 3.0; p->penalM = 1.0;
+    p->dalpha = 1e-2; p->dbeta = 1e-4;
+    p->ti = 1; p->nsteps = 100; p->t1 = 1.0;
+    p->objective = 3; p->boundaries = 1; p->loadloc = 2; p->loadsignal = 1; p->pulse_steps = 20; p->region = 2;
+    p->nlvls = 3; p->precision = TT_PREC_FP64; p->rtol = 1e-8; p->maxit = 200; p->cheb_degree = 2;
+    p->checkpoint_slots = 0;
+    p->rmin = 0.08; p->filter_rtol = 1e-10;
+    p->verbose = 0;
+    return TT_OK;
+}
+
+int tt_create(const tt_params* prm, tt_ctx** out) {
+    if (!prm || !out) return TT_ERR_ARG;
+    *out = nullptr;
+    try {
+        auto h = new tt_ctx;
+        h->c.reset(new Context(*prm));
+        *out = h;
+    } catch (const Error& e) {
+        g_create_error = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TT_ERR_INTERNAL;
+    }
+    return TT_OK;
+}
+
+void tt_destroy(tt_ctx* ctx) { delete ctx; }
+
+const char* tt_last_error(const tt_ctx* ctx) {
+    if (!ctx || !ctx->c) return g_create_error.c_str();
+    return ctx->c->last_error.c_str();
+}
+
+long tt_num_nodes(const tt_ctx* ctx) { return ctx && ctx->c ? ctx->c->g.nn : -1; }
+long tt_num_elements(const tt_ctx* ctx) { return ctx && ctx->c ? ctx->c->g.nel : -1; }
+
+int tt_set_density(tt_ctx* ctx, const double* xphys, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(xphys, TT_ERR_ARG, "null xphys");
+    const long nel = C.g.nel;
+    if (on_device) tt::launch_copy(xphys, C.xphys.p, nel, true, true, C.stream);
+    else TT_HIP(hipMemcpyAsync(C.xphys.p, xphys, nel * 8, hipMemcpyHostToDevice, C.stream));
+    tt::launch_sum(C.xphys.p, nel, C.red, tt::SC_TMP3, C.stream);
+    const double s = C.read_scal(tt::SC_TMP3);
+    TT_CHECK(std::isfinite(s), TT_ERR_NONFINITE, "non-finite density");
+    const tt_params& p = C.prm;
+    tt::launch_interp(C.xphys.p, nel, p.sInt, p.Emin, p.Emax, p.penalK, p.mInt, p.Rmin, p.Rmax, p.penalM, C.P.p,
+                      C.kf.p, C.mf.p, C.dkf.p, C.dmf.p, C.Pkf.p, C.Pmf.p, C.stream);
+    tt::launch_convert_coef(C.kf.p, C.kf32.p, nel, tt::COEF_F32, C.stream);
+    tt::launch_convert_coef(C.mf.p, C.mf32.p, nel, tt::COEF_F32, C.stream);
+    C.mg.set_coefficients(C.kf.p, C.mf.p);
+    C.sync();
+    C.density_set = true;
+    C.initial_done = false;
+    C.forward_done = false;
+    TT_API_END
+}
+
+int tt_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(C.density_set, TT_ERR_STATE, "tt_set_density must be called first");
+    TT_CHECK(u && y && (prec == 0 || prec == 1) && which >= 0 && which <= 2, TT_ERR_ARG, "bad tt_op_apply arguments");
+    double cK = which == TT_OP_K ? 1.0 : which == TT_OP_M ? 0.0 : C.cK_A;
+    double cM = which == TT_OP_K ? 0.0 : which == TT_OP_M ? 1.0 : C.cM_A;
+    const long nd = C.g.nd;
+    const size_t es = prec == 0 ? 8 : 4;
+    tt::OpLevel op = C.fine_op(prec == 0);
+    if (on_device) {
+        tt::launch_apply_bc(op, cK, cM, u, y, prec == 0, C.stream);
+    } else {
+        DBuf<char> du(nd * es), dy(nd * es);
+        TT_HIP(hipMemcpyAsync(du.p, u, nd * es, hipMemcpyHostToDevice, C.stream));
+        tt::launch_apply_bc(op, cK, cM, du.p, dy.p, prec == 0, C.stream);
+        TT_HIP(hipMemcpyAsync(y, dy.p, nd * es, hipMemcpyDeviceToHost, C.stream));
+        C.sync();
+    }
+    TT_API_END
+}
+
+int tt_complete_initial_state(tt_ctx* ctx) {
+    TT_API_BEGIN(ctx)
+    C.complete_initial_state();
+    C.sync();
+    TT_API_END
+}
+
+int tt_forward(tt_ctx* ctx) {
+    TT_API_BEGIN(ctx)
+    C.forward();
+    TT_API_END
+}
+
+int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(t >= 0 && t <= C.prm.nsteps, TT_ERR_ARG, "time step out of range");
+    int k = -1;
+    for (size_t i = 0; i < C.slot_step.size(); i++)
+        if (C.slot_step[i] == t) k = (int)i;
+    TT_CHECK(k >= 0, TT_ERR_STATE, "state not stored (use checkpoint_slots >= nsteps to keep every state)");
+    const long nd = C.g.nd;
+    double* outs[3] = {u, v, a};
+    for (int q = 0; q < 3; q++) {
+        if (!outs[q]) continue;
+        TT_HIP(hipMemcpyAsync(outs[q], C.slots[k].p + q * nd, nd * 8,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, C.stream));
+    }
+    C.sync();
+    TT_API_END
+}
+
+int tt_objective_gradient(tt_ctx* ctx, double* fx, double* dfdx, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(fx, TT_ERR_ARG, "null fx");
+    DevOut o(C, dfdx, C.g.nel, on_device);
+    C.objective_gradient(fx, o.p);
+    o.finish();
+    TT_API_END
+}
+
+int tt_volume_constraint(tt_ctx* ctx, const double* xphys, double volfrac, double* gx, double* dgdx,
+                         int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(xphys && gx, TT_ERR_ARG, "null argument");
+    const long nel = C.g.nel;
+    DevIn xi(C, xphys, nel, on_device);
+    tt::launch_sum(xi.p, nel, C.red, tt::SC_TMP3, C.stream);
+    const double s = C.read_scal(tt::SC_TMP3);
+    *gx = s / (double)nel - volfrac;
+    if (dgdx) {
+        DevOut o(C, dgdx, nel, on_device);
+        tt::launch_fill(o.p, 1.0 / (double)nel, nel, true, C.stream);
+        o.finish();
+    }
+    TT_API_END
+}
+
+int tt_pde_filter(tt_ctx* ctx, const double* in, double* out, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(in && out, TT_ERR_ARG, "null argument");
+    const long nel = C.g.nel;
+    DevIn xi(C, in, nel, on_device);
+    DevOut o(C, out, nel, on_device);
+    C.pde_filter(xi.p, C.ftmp_e.p);
+    tt::launch_copy(C.ftmp_e.p, o.p, nel, true, true, C.stream);
+    o.finish();
+    TT_API_END
+}
+
+int tt_filter_project(tt_ctx* ctx, const double* x, double* xtilde, double* xphys, int projection, double beta,
+                      double eta, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(x && xtilde && xphys, TT_ERR_ARG, "null argument");
+    const long nel = C.g.nel;
+    DevIn xi(C, x, nel, on_device);
+    DevOut ot(C, xtilde, nel, on_device), op(C, xphys, nel, on_device);
+    C.pde_filter(xi.p, C.ftmp_e.p);
+    tt::launch_clamp_project(C.ftmp_e.p, ot.p, op.p, nel, projection, beta, eta, C.stream);
+    ot.finish();
+    op.finish();
+    TT_API_END
+}
+
+int tt_filter_gradients(tt_ctx* ctx, const double* xtilde, double* dfdx, double* dgdx, int projection, double beta,
+                        double eta, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(xtilde && dfdx, TT_ERR_ARG, "null argument");
+    const long nel = C.g.nel;
+    DevIn xt(C, xtilde, nel, on_device);
+    DevOut od(C, dfdx, nel, on_device, true), og(C, dgdx, nel, on_device, true);
+    if (projection) tt::launch_project_chain(xt.p, od.p, og.p, nel, beta, eta, C.stream);
+    C.pde_filter(od.p, C.ftmp_e.p);
+    tt::launch_copy(C.ftmp_e.p, od.p, nel, true, true, C.stream);
+    if (og.p) {
+        C.pde_filter(og.p, C.ftmp_e.p);
+        tt::launch_copy(C.ftmp_e.p, og.p, nel, true, true, C.stream);
+    }
+    od.finish();
+    og.finish();
+    TT_API_END
+}
+
+int tt_mnd(tt_ctx* ctx, const double* x, double* mnd, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(x && mnd, TT_ERR_ARG, "null argument");
+    const long nel = C.g.nel;
+    DevIn xi(C, x, nel, on_device);
+    tt::launch_mnd(xi.p, nel, C.red, tt::SC_TMP3, C.stream);
+    *mnd = C.read_scal(tt::SC_TMP3) / (double)nel;
+    TT_API_END
+}
+
+int tt_mma_create(tt_ctx* ctx, double a, double c, double d, tt_mma** out) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(out, TT_ERR_ARG, "null output");
+    auto m = new tt_mma;
+    m->ctx = &C;
+    m->n = C.g.nel;
+    m->a = a;
+    m->c = c;
+    m->d = d;
+    for (DBuf<double>* b : {&m->L, &m->U, &m->alpha, &m->beta, &m->p0, &m->q0, &m->pij, &m->qij, &m->xo1, &m->xo2}) {
+        b->alloc(m->n);
+        TT_HIP(hipMemsetAsync(b->p, 0, m->n * 8, C.stream));
+    }
+    C.sync();
+    *out = m;
+    TT_API_END
+}
+
+void tt_mma_destroy(tt_mma* m) { delete m; }
+
+#define TT_MMA_BEGIN(m)                 \
+    if (!(m) || !(m)->ctx) return TT_ERR_ARG; \
+    Context& C = *(m)->ctx;             \
+    try {
+
+int tt_mma_set_asymptotes(tt_mma* m, double init, double dec, double inc) {
+    TT_MMA_BEGIN(m)
+    m->asyminit = init;
+    m->asymdec = dec;
+    m->asyminc = inc;
+    TT_API_END
+}
+
+int tt_mma_set_outer_movelimit(tt_mma* m, double Xmin, double Xmax, double movlim, const double* x, double* xmin,
+                               double* xmax, int on_device) {
+    TT_MMA_BEGIN(m)
+    TT_CHECK(x && xmin && xmax, TT_ERR_ARG, "null argument");
+    DevIn xi(C, x, m->n, on_device);
+    DevOut a(C, xmin, m->n, on_device), b(C, xmax, m->n, on_device);
+    tt::launch_mma_movelimit(xi.p, a.p, b.p, m->n, Xmin, Xmax, movlim, C.stream);
+    a.finish();
+    b.finish();
+    TT_API_END
+}
+
+int tt_mma_update(tt_mma* m, double* x, const double* dfdx, const double* gx, const double* dgdx, const double* xmin,
+                  const double* xmax, int on_device) {
+    TT_MMA_BEGIN(m)
+    TT_CHECK(x && dfdx && gx && dgdx && xmin && xmax, TT_ERR_ARG, "null argument");
+    const long n = m->n;
+    DevOut xo(C, x, n, on_device, true);
+    DevIn df(C, dfdx, n, on_device), dg(C, dgdx, n, on_device), lo(C, xmin, n, on_device), hi(C, xmax, n, on_device);
+    m->k++;  // GenSub (MMA.cc:487-600)
+    tt::launch_mma_gensub(m->dev(), xo.p, df.p, dg.p, lo.p, hi.p, m->k, m->asyminit, m->asymdec, m->asyminc, C.red,
+                          C.stream);
+    m->b = C.read_scal(tt::SC_TMP0) + (-gx[0]);
+    tt::mma_solve_dip(m, xo.p);
+    xo.finish();
+    TT_API_END
+}
+
+int tt_mma_design_change(tt_mma* m, const double* x, double* xold, double* ch, int on_device) {
+    TT_MMA_BEGIN(m)
+    TT_CHECK(x && xold && ch, TT_ERR_ARG, "null argument");
+    DevIn xi(C, x, m->n, on_device);
+    DevOut xo(C, xold, m->n, on_device, true);
+    tt::launch_mma_change(xi.p, xo.p, m->n, C.red, tt::SC_TMP3, C.stream);
+    *ch = C.read_scal(tt::SC_TMP3);
+    xo.finish();
+    TT_API_END
+}
+
+int tt_mma_kkt_residual(tt_mma* m, const double* x, const double* dfdx, const double* gx, const double* dgdx,
+                        const double* xmin, const double* xmax, double* norm2, double* normInf, int on_device) {
+    TT_MMA_BEGIN(m)
+    TT_CHECK(x && dfdx && gx && dgdx && xmin && xmax && norm2 && normInf, TT_ERR_ARG, "null argument");
+    const long n = m->n;
+    DevIn xi(C, x, n, on_device), df(C, dfdx, n, on_device), dg(C, dgdx, n, on_device), lo(C, xmin, n, on_device),
+        hi(C, xmax, n, on_device);
+    tt::launch_mma_kkt(xi.p, df.p, dg.p, lo.p, hi.p, n, m->lam, C.red, C.stream);
+    TT_HIP(hipMemcpyAsync(C.red.host, C.red.scal.p, sizeof(double) * tt::SC_COUNT, hipMemcpyDeviceToHost, C.stream));
+    C.sync();
+    double n2 = C.red.host[tt::SC_TMP1], ni = C.red.host[tt::SC_TMP2];
+    const double ri = m->lam * (m->a * m->z + m->y - gx[0]);  // MMA.cc:476-482
+    n2 += std::pow(ri, 2.0);
+    ni = std::max(std::fabs(ri), ni);
+    *norm2 = std::sqrt(n2);
+    *normInf = ni;
+    TT_API_END
+}
+
+int tt_mma_get_asymptotes(tt_mma* m, double* L, double* U, double* lam) {
+    TT_MMA_BEGIN(m)
+    if (L) TT_HIP(hipMemcpyAsync(L, m->L.p, m->n * 8, hipMemcpyDeviceToHost, C.stream));
+    if (U) TT_HIP(hipMemcpyAsync(U, m->U.p, m->n * 8, hipMemcpyDeviceToHost, C.stream));
+    C.sync();
+    if (lam) *lam = m->lam;
+    TT_API_END
+}
+
+int tt_get_stats(tt_ctx* ctx, tt_stats* st) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(st, TT_ERR_ARG, "null stats");
+    *st = C.stats;
+    TT_API_END
+}
+
+int tt_reset_stats(tt_ctx* ctx) {
+    TT_API_BEGIN(ctx)
+    double lm[8];
+    std::memcpy(lm, C.stats.lambda_max, sizeof(lm));
+    const int nl = C.stats.nlvls;
+    C.stats = tt_stats{};
+    std::memcpy(C.stats.lambda_max, lm, sizeof(lm));
+    C.stats.nlvls = nl;
+    TT_API_END
+}
+
+int tt_synchronize(tt_ctx* ctx) {
+    TT_API_BEGIN(ctx)
+    C.sync();
+    TT_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// Library context: device data of one transient-elasticity problem on one GPU.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/topopt_gpu.h"
+#include "tt_internal.h"
+
+namespace tt {
+
+struct Context;
+
+// One multigrid level of the elasticity operator.
+struct MGLevel {
+    Grid g;
+    bool fp64 = true;                  // precision of this level's vectors and element matrices
+    DBuf<uint8_t> mask;
+    DBuf<double> kf64, mf64;           // FP64 coefficients (coarse levels: averages of children)
+    DBuf<float> kf32, mf32;            // mixed mode, finest level
+    DBuf<bf16_t> kfbf, mfbf;           // mixed mode, coarse levels
+    DBuf<double> KE64, M864;
+    DBuf<float> KE32, M832;
+    OpLevel op{};
+    DBuf<char> b, x, x2, r, d, d2, dinv;  // level vectors (element size = 8 or 4 bytes)
+    double lmax = 1.0;
+    size_t esz() const { return fp64 ? 8 : 4; }
+};
+
+// Geometric multigrid preconditioner (V-cycle, Chebyshev smoothing on D^-1 A,
+// rediscretised coarse operators, dense FP64 coarsest solve).
+struct Multigrid {
+    Context* ctx = nullptr;
+    std::vector<std::unique_ptr<MGLevel>> L;
+    bool mixed = false;
+    int cheb_deg = 2;
+    double cK = 1.0, cM = 0.0;
+    double cheb_lo = 0.1, cheb_hi = 1.1;
+    // coarsest level
+    int nc = 0;
+    bool dense = true;
+    DBuf<double> Ainv, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest vectors
+    int coarse_deg = 12;
+    double clmin = 0.0, clmax = 1.0;
+
+    void build(Context* c, int nlvls, bool mixed_prec, int deg);
+    void set_coefficients(const double* kf_fine, const double* mf_fine);
+    void setup_operator(double cK_, double cM_);
+    // z = B r (FP64 in and out)
+    void apply(const double* r, double* z);
+
+   private:
+    void vcycle(int l);
+    void coarse_solve();
+    double power_lmax(int l, int iters);
+    void assemble_coarse_inverse();
+    OpLevel coarse_op64() const;
+};
+
+struct SolveResult {
+    int iters = 0;
+    double relres = 0.0;
+};
+
+struct Context {
+    tt_params prm{};
+    Grid g{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    Reducer red;
+    std::string last_error;
+
+    double KE[576], M8[64];
+    double a[6], dt = 0.0;
+    double cK_A = 1.0, cM_A = 1.0;  // Newmark operator scalars
+
+    // element data
+    DBuf<double> xphys, kf, mf, dkf, dmf, P, Pkf, Pmf;
+    DBuf<double> rcnt;  // per node: sum of region indicators of the adjacent elements
+    DBuf<double> F;     // load location vector (3 nn)
+    DBuf<uint8_t> mask;
+    std::vector<double> Fhost;
+    OpLevel op_fine64{};  // FP64 fine operator view (for FCG residuals)
+    DBuf<double> KE64d, M864d;
+    DBuf<float> KE32d, M832d;
+    DBuf<float> kf32, mf32;  // FP32 fine coefficients for the FP32 microbenchmark apply
+    bool density_set = false;
+    bool initial_done = false;
+    bool forward_done = false;
+
+    Multigrid mg;
+
+    // Krylov work vectors (FP64, nd)
+    DBuf<double> kr, krp, kz, kp0, kp1, kq, dinvM;
+
+    // time states: slot storage (u, v, a contiguous, 3*nd each)
+    std::vector<DBuf<double>> slots;
+    std::vector<int> slot_step;  // time step held by each slot (-1 = free)
+    bool store_all = false;
+    DBuf<double> st_cur, st_next;
+
+    // adjoint work
+    DBuf<double> lam, lamn, lamT, mu, nu, mun, nun, hdu, hddu, hdddu, Kl, Ml, rhs, df, fxs, nu0;
+
+    // PDE filter
+    double KF[64];
+    DBuf<double> KFd, fdinv, fU, frhs, fr, fz, fp, fq, ftmp_e, ftmp_e2;
+    Grid fg{};
+
+    // stats
+    tt_stats stats{};
+
+    explicit Context(const tt_params& p);
+    ~Context();
+
+    // helpers
+    OpLevel fine_op(bool fp64) const;
+    SolveResult solve_A(const double* b, double* x);         // Newmark operator, MG-preconditioned FCG
+    SolveResult solve_M(const double* b, double* x);         // mass operator N M N + I - N, Jacobi FCG
+    SolveResult fcg(double cK, double cM, bool use_mg, const double* b, double* x);
+    void sync();
+    double read_scal(int slot);
+
+    // transient
+    void complete_initial_state();
+    void forward();
+    void objective_gradient(double* fx_host, double* dfdx_dev);
+    void newmark_step(const double* s_in, double* s_out, int t);
+    double load_intensity(int step) const;
+
+    // filter
+    void pde_filter(const double* in_dev, double* out_dev);
+};
+
+}  // namespace tt

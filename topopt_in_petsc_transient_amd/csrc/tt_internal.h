@@ -1,0 +1,255 @@
+// Internal declarations of the MI355X transient-elasticity library.
+//
+// Layout in HBM (natural ordering of the reference's single-rank DMDA):
+//   node  n = i + nx*(j + ny*k)      vectors: 3 dofs per node, interleaved (3n+c)
+//   elem  e = i + ex*(j + ey*k)      per-element coefficients kfac/mfac
+// Each multigrid level l holds its own element coefficients, Dirichlet mask
+// (one byte per node, bit c set when dof c is clamped), inverse diagonal and
+// vectors.  Precision of level vectors is FP64 in the FP64 mode and FP32 in
+// the mixed mode; coarse-level coefficients are BF16 in the mixed mode.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace tt {
+
+typedef uint16_t bf16_t;
+
+// ------------------------------------------------------------------ errors
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define TT_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw ::tt::Error{-2, std::string("HIP error ") + hipGetErrorString(e_) + " at " +   \
+                                      __FILE__ + ":" + std::to_string(__LINE__)};               \
+    } while (0)
+
+#define TT_CHECK(cond, code, msg)                      \
+    do {                                               \
+        if (!(cond)) throw ::tt::Error{(code), (msg)}; \
+    } while (0)
+
+// ------------------------------------------------------------------ device buffers
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t n_) { alloc(n_); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t n_) {
+        release();
+        n = n_;
+        if (n) TT_HIP(hipMalloc((void**)&p, sizeof(T) * n));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* get() const { return p; }
+};
+
+// ------------------------------------------------------------------ grid
+struct Grid {
+    int ex, ey, ez;  // elements
+    int nx, ny, nz;  // nodes
+    long nel, nn, nd;
+    double hx, hy, hz;
+};
+
+inline Grid make_grid(int ex, int ey, int ez, double lx, double ly, double lz) {
+    Grid g;
+    g.ex = ex; g.ey = ey; g.ez = ez;
+    g.nx = ex + 1; g.ny = ey + 1; g.nz = ez + 1;
+    g.nel = (long)ex * ey * ez;
+    g.nn = (long)g.nx * g.ny * g.nz;
+    g.nd = 3 * g.nn;
+    g.hx = lx / ex; g.hy = ly / ey; g.hz = lz / ez;
+    return g;
+}
+
+// ------------------------------------------------------------------ host element matrices
+void hex8_stiffness(double hx, double hy, double hz, double nu, double* ke);  // 24x24, E = 1
+void hex8_mass(double hx, double hy, double hz, double* m8);                 // 8x8 scalar, density 1
+void hex8_helmholtz(double hx, double hy, double hz, double R, double* kf);  // 8x8, R^2 Lap + mass
+
+// ------------------------------------------------------------------ reductions
+// Deterministic device reductions: every reducing kernel writes one partial per
+// block, and the last block to finish sums the partials in block order.
+struct Reducer {
+    DBuf<double> partials;   // [maxblocks * 8]
+    DBuf<unsigned> counter;  // [8]
+    DBuf<double> scal;       // device scalars (see SC_* indices)
+    double* host = nullptr;  // pinned mirror of scal
+    int maxblocks = 0;
+    void init(int maxb);
+    ~Reducer();
+};
+
+enum {
+    SC_GAMMA = 0,   // z.r of current iteration
+    SC_GAMMA_OLD,   // z.r of previous iteration
+    SC_DELTA,       // z.r_prev
+    SC_BETA,
+    SC_PQ,
+    SC_ALPHA,
+    SC_RR,          // r.r
+    SC_TMP0,
+    SC_TMP1,
+    SC_TMP2,
+    SC_TMP3,
+    SC_FX0,         // objective accumulators start here
+    SC_COUNT = 32
+};
+
+// ------------------------------------------------------------------ kernel launchers (tt_kernels.hip)
+// Elasticity operator on one level.  Coefficient storage CT is double, float or BF16.
+enum CoefKind { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2 };
+
+struct OpLevel {
+    Grid g;
+    int coef_kind;         // CoefKind
+    const void* kf;        // element stiffness factors (kfac)
+    const void* mf;        // element mass factors (mfac)
+    const double* KE64;    // 576
+    const double* M864;    // 64
+    const float* KE32;
+    const float* M832;
+    const uint8_t* mask;   // per node clamped bits
+};
+
+// y = N.(cK*K + cM*M)(N.u) on free dofs, identity on clamped dofs.  Optional dot u.y into scal[dot_slot].
+void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void* y, bool fp64, hipStream_t s,
+                     Reducer* red = nullptr, int dot_slot = -1);
+// r = b - A x
+void launch_residual(const OpLevel& L, double cK, double cM, const void* b, const void* x, void* r, bool fp64,
+                     hipStream_t s);
+// Chebyshev step: r = b - A x; dn = c0*d + c1*dinv.r; xn = x + dn  (d may be null => c0 ignored)
+void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
+                      const void* dinv, double c0, double c1, void* xn, void* dn, bool fp64, hipStream_t s);
+// split outputs: yk = N.(cK K (N.u)), ym = N.(cM M (N.u))  (FP64)
+void launch_apply_split(const OpLevel& L, double cK, double cM, const double* u, double* yk, double* ym,
+                        hipStream_t s);
+// unmasked K/M apply with arbitrary coefficient arrays (FP64): y = cK*K(kf)u + cM*M(mf)u
+void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, double cK, double cM, const double* u,
+                      double* y, hipStream_t s);
+// Newmark right-hand side: R = N.( fs*F + sum_i (K-weights, M-weights) applied to u,v,a )
+void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, const double* a, const double* F,
+                        double fs, const double wK[3], const double wM[3], double* R, hipStream_t s);
+// diagonal of (cK K + cM M) with identity on clamped dofs, inverted
+void launch_diag_inv(const OpLevel& L, double cK, double cM, void* dinv, bool fp64, hipStream_t s);
+
+// FCG fused apply: p = z + beta*p_old (beta from scal), q = A p, partial p.q -> alpha = gamma / pq
+void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
+                      double* q, Reducer& red, bool first, hipStream_t s);
+
+// vector kernels
+void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s);
+void launch_copy(const void* src, void* dst, long n, bool src64, bool dst64, hipStream_t s);
+void launch_cheb_first(const void* b, const void* dinv, double c1, void* x, long n, bool fp64, hipStream_t s);
+void launch_restrict(const Grid& fine, const Grid& coarse, const void* rf, const uint8_t* cmask, void* bc,
+                     bool fp64, hipStream_t s);
+void launch_prolong_add(const Grid& fine, const Grid& coarse, const void* xc, const uint8_t* fmask, void* xf,
+                        bool fp64, hipStream_t s);
+void launch_coarsen_coef(const Grid& fine, const Grid& coarse, const double* kf_f, const double* mf_f, double* kf_c,
+                         double* mf_c, hipStream_t s);
+void launch_convert_coef(const double* src, void* dst, long n, int kind, hipStream_t s);
+void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, double pK, int mInt, double Rmin,
+                   double Rmax, double pM, const double* P, double* kf, double* mf, double* dkf, double* dmf,
+                   double* Pkf, double* Pmf, hipStream_t s);
+void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hipStream_t s);
+
+// dots (FP64): results land in red.scal[slot..]
+void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s);
+void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
+                     hipStream_t s);
+void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
+                       hipStream_t s);
+void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipStream_t s);
+void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s);
+void launch_scale_by_scal(void* x, long n, const double* scal, int slot, bool fp64, hipStream_t s);
+
+// Newmark corrector: vn = a0 v + a1 a + a2 (un - u); an = -a3 v - a4 a + a5 (un - u)
+void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,
+                           double* an, const double* c, long n, hipStream_t s);
+// adjoint combination (see transient.cpp)
+void launch_adjoint_combine(const double* Kl, const double* Ml, const double* hdu, const double* hddu,
+                            const double* hdddu, const double* mun, const double* nun, const uint8_t* mask,
+                            const double* coef, bool terminal, double* mu, double* nu, double* rhs, long nn,
+                            hipStream_t s);
+void launch_adjoint0_rhs(const double* Kl, const double* Ml, const double* hdddu, const double* mun,
+                         const double* nun, const uint8_t* mask, const double* coef, double* rhs, long nn,
+                         hipStream_t s);
+// objective gradient wrt state for objectives 0..2 (diagonal with region counts)
+void launch_hdiag(const double* w, const double* cnt, double dt, double* h, long nn, hipStream_t s);
+void launch_region_count(const Grid& g, const double* P, double* cnt, hipStream_t s);
+
+// sensitivities and objective (per time step); accumulates into df and red.scal[slot] (objective)
+struct SensArgs {
+    Grid g;
+    const double* KE;  // 576
+    const double* M8;  // 64
+    const double *kf, *mf, *dkf, *dmf, *P;
+    const double *lam, *u, *v, *a, *ub, *vb, *ab;
+    double wK[4];  // coefficients of (u, ub, vb, ab) in w_K
+    double wM[4];  // coefficients of (u, ub, vb, ab) in w_M
+    double dt, dalpha, dbeta;
+    int objective;
+    int mode;      // 0: general step (lambda), 1: t = 0 step (nu0 with dK u0 + dC v0 + dM a0)
+    double* df;
+};
+void launch_sens(const SensArgs& a, Reducer& red, double* fx_out, hipStream_t s);
+
+// PDE filter (scalar Helmholtz on nodes)
+void launch_filter_apply(const Grid& g, const double* kf, const double* u, double* y, Reducer* red, int dot_slot,
+                         hipStream_t s);
+void launch_filter_T(const Grid& g, const double* x, double scale, double* rhs, hipStream_t s);
+void launch_filter_Tt(const Grid& g, const double* U, double* out, hipStream_t s);
+void launch_filter_diag(const Grid& g, const double* kf, double* dinv, hipStream_t s);
+void launch_clamp_project(const double* xt_in, double* xt, double* xp, long n, int projection, double beta,
+                          double eta, hipStream_t s);
+void launch_project_chain(const double* xt, double* df, double* dg, long n, double beta, double eta,
+                          hipStream_t s);
+// reductions over element arrays
+void launch_sum(const double* x, long n, Reducer& red, int slot, hipStream_t s);
+void launch_mnd(const double* x, long n, Reducer& red, int slot, hipStream_t s);
+
+// MMA kernels (m = 1)
+struct MMADev {
+    long n;
+    double *L, *U, *alpha, *beta, *p0, *q0, *pij, *qij, *xo1, *xo2;
+};
+void launch_mma_gensub(const MMADev& m, const double* x, const double* df, const double* dg, const double* xmin,
+                       const double* xmax, int k, double asyminit, double asymdec, double asyminc, Reducer& red,
+                       hipStream_t s);
+void launch_mma_xyz_grad_hess(const MMADev& m, double lam, double* x, Reducer& red, hipStream_t s);
+void launch_mma_xyz_res(const MMADev& m, double lam, double* x, Reducer& red, hipStream_t s);
+void launch_mma_movelimit(const double* x, double* xmin, double* xmax, long n, double Xmin, double Xmax,
+                          double movlim, hipStream_t s);
+void launch_mma_change(const double* x, double* xold, long n, Reducer& red, int slot, hipStream_t s);
+void launch_mma_kkt(const double* x, const double* df, const double* dg, const double* xmin, const double* xmax,
+                    long n, double lam, Reducer& red, hipStream_t s);
+void launch_fill_scal(void* x, long n, double v, hipStream_t s);
+void launch_scale(double* x, long n, double a, hipStream_t s);
+// out = a*x + b*y (out may alias x or y)
+void launch_lincomb(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
+
+}  // namespace tt

@@ -1,0 +1,1032 @@
+// HIP kernels for gfx950: matrix-free SIMP hex8 operator, multigrid transfers,
+// Krylov vector kernels and deterministic reductions.
+//
+// The operator y = sum_e (cK*kfac_e*KE + cM*mfac_e*ME) u_e of the reference's
+// assembled AIJ matrices (LinearElasticity.cc:2502-2619) is applied node by
+// node: each thread owns one node and gathers the contributions of its (up to)
+// eight elements from an LDS tile holding the node box (8x8x4 owned + halo 1)
+// and the element coefficients.  No scatter, no atomics: results are
+// bit-reproducible.  Epilogue functors fuse the Krylov / Chebyshev vector work
+// into the same pass.
+#include "tt_internal.h"
+
+namespace tt {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ double coef_ld(const double* p, long i) { return p[i]; }
+__device__ __forceinline__ double coef_ld(const float* p, long i) { return (double)p[i]; }
+__device__ __forceinline__ double coef_ld(const bf16_t* p, long i) {
+    return (double)__uint_as_float(((uint32_t)p[i]) << 16);
+}
+
+constexpr int BS = 256;  // threads per block for all 1-D kernels
+
+// Element-local corner numbering of the reference (LinearElasticity.cc:3067-3074)
+__host__ __device__ constexpr int corner_id(int x, int y, int z) { return z * 4 + (y == 0 ? (x == 0 ? 0 : 1) : (x == 1 ? 2 : 3)); }
+__host__ __device__ constexpr int cx_of(int b) { return (b & 3) == 1 || (b & 3) == 2 ? 1 : 0; }
+__host__ __device__ constexpr int cy_of(int b) { return (b & 3) >= 2 ? 1 : 0; }
+__host__ __device__ constexpr int cz_of(int b) { return b >> 2; }
+
+// ------------------------------------------------------------------ deterministic block reduction
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
+    // sh: BS/64 * NV doubles
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[k] = x;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[wid * NV + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w * NV + k];
+            v[k] = s;
+        }
+    }
+}
+
+// Writes the block's NV partials; the last block sums all partials in block
+// order and hands the totals to fin(totals) (thread 0 only).
+template <int NV, class Fin>
+__device__ __forceinline__ void reduce_finish(double (&v)[NV], double* partials, unsigned* counter, Fin fin) {
+    __shared__ double sh[16 * NV];
+    __shared__ bool last;
+    block_sum<NV>(v, sh);
+    const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[(size_t)bid * NV + k] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        last = (t == nblk - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    block_sum<NV>(acc, sh);
+    if (threadIdx.x == 0) {
+        fin(acc);
+        *counter = 0u;
+    }
+}
+
+struct FinStore1 {
+    double* out;
+    __device__ void operator()(const double (&t)[1]) const { out[0] = t[0]; }
+};
+struct FinStore2 {
+    double* out;
+    __device__ void operator()(const double (&t)[2]) const { out[0] = t[0]; out[1] = t[1]; }
+};
+
+void Reducer::init(int maxb) {
+    maxblocks = maxb;
+    partials.alloc((size_t)maxb * 8);
+    counter.alloc(8);
+    scal.alloc(SC_COUNT);
+    TT_HIP(hipMemset(counter.p, 0, sizeof(unsigned) * 8));
+    TT_HIP(hipMemset(scal.p, 0, sizeof(double) * SC_COUNT));
+    TT_HIP(hipHostMalloc((void**)&host, sizeof(double) * SC_COUNT, hipHostMallocDefault));
+}
+Reducer::~Reducer() {
+    if (host) (void)hipHostFree(host);
+}
+
+static inline int nblocks(long n, int per = BS) { return (int)((n + per - 1) / per); }
+static inline int red_blocks(long n) {
+    int b = nblocks(n);
+    return b > 2048 ? 2048 : (b < 1 ? 1 : b);
+}
+
+// ------------------------------------------------------------------ operator apply
+constexpr int TX = 8, TY = 8, TZ = 4, NT = TX * TY * TZ;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HN = HX * HY * HZ;
+constexpr int EXs = TX + 1, EYs = TY + 1, EZs = TZ + 1, EN = EXs * EYs * EZs;
+static_assert(NT == BS, "tile must match block size");
+
+struct GridArgs {
+    int nx, ny, nz, ex, ey, ez;
+};
+static inline GridArgs gargs(const Grid& g) { return GridArgs{g.nx, g.ny, g.nz, g.ex, g.ey, g.ez}; }
+
+// Per-node contraction: yK = sum_e kfe * KE[a rows] u_e, yM = sum_e mfe * (M8 (x) I3)[a rows] w_e
+template <typename T, bool TWO>
+__device__ __forceinline__ void node_contract(const T* __restrict__ sU, const T* __restrict__ sW,
+                                              const T* __restrict__ sk, const T* __restrict__ sm,
+                                              const T* __restrict__ KE, const T* __restrict__ M8, int tx, int ty,
+                                              int tz, T (&yK)[3], T (&yM)[3]) {
+#pragma unroll
+    for (int ez = 0; ez < 2; ez++)
+#pragma unroll
+        for (int ey = 0; ey < 2; ey++)
+#pragma unroll
+            for (int ex = 0; ex < 2; ex++) {
+                const int a = corner_id(1 - ex, 1 - ey, 1 - ez);
+                const int eidx = ((tz + ez) * EYs + (ty + ey)) * EXs + (tx + ex);
+                const T kfe = sk[eidx], mfe = sm[eidx];
+                T k0 = 0, k1 = 0, k2 = 0, m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const int nidx = ((tz + ez + cz_of(b)) * HY + (ty + ey + cy_of(b))) * HX + (tx + ex + cx_of(b));
+                    const T u0 = sU[3 * nidx], u1 = sU[3 * nidx + 1], u2 = sU[3 * nidx + 2];
+                    const T* kr = KE + (3 * a) * 24 + 3 * b;
+                    k0 = fma(kr[0], u0, fma(kr[1], u1, fma(kr[2], u2, k0)));
+                    k1 = fma(kr[24], u0, fma(kr[25], u1, fma(kr[26], u2, k1)));
+                    k2 = fma(kr[48], u0, fma(kr[49], u1, fma(kr[50], u2, k2)));
+                    const T mab = M8[a * 8 + b];
+                    if (TWO) {
+                        m0 = fma(mab, sW[3 * nidx], m0);
+                        m1 = fma(mab, sW[3 * nidx + 1], m1);
+                        m2 = fma(mab, sW[3 * nidx + 2], m2);
+                    } else {
+                        m0 = fma(mab, u0, m0);
+                        m1 = fma(mab, u1, m1);
+                        m2 = fma(mab, u2, m2);
+                    }
+                }
+                yK[0] = fma(kfe, k0, yK[0]);
+                yK[1] = fma(kfe, k1, yK[1]);
+                yK[2] = fma(kfe, k2, yK[2]);
+                yM[0] = fma(mfe, m0, yM[0]);
+                yM[1] = fma(mfe, m1, yM[1]);
+                yM[2] = fma(mfe, m2, yM[2]);
+            }
+}
+
+// Input staging policies: produce the (masked) value of dof (node n, comp c) for the K and M inputs
+template <typename T>
+struct StageOne {  // w = u
+    const T* u;
+    __device__ __forceinline__ T k(long i) const { return u[i]; }
+    __device__ __forceinline__ T m(long i) const { return u[i]; }
+};
+template <typename T>
+struct StageFcg {  // p = z + beta*p_old, beta read from device scalar
+    const T* z;
+    const T* pold;
+    const double* beta;  // null on the first iteration
+    __device__ __forceinline__ T k(long i) const { return beta ? z[i] + (T)(*beta) * pold[i] : z[i]; }
+    __device__ __forceinline__ T m(long i) const { return k(i); }
+};
+struct StageThree {  // wK = sum_i cK_i in_i, wM = sum_i cM_i in_i (FP64)
+    const double *u, *v, *a;
+    double k0, k1, k2, m0, m1, m2;
+    __device__ __forceinline__ double k(long i) const { return k0 * u[i] + k1 * v[i] + k2 * a[i]; }
+    __device__ __forceinline__ double m(long i) const { return m0 * u[i] + m1 * v[i] + m2 * a[i]; }
+};
+
+template <typename T, typename CT, bool TWO, bool MASKIN, class Stage, class Epi>
+__global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ KE, const T* __restrict__ M8,
+                                              const CT* __restrict__ kf, const CT* __restrict__ mf, T cK, T cM,
+                                              const uint8_t* __restrict__ mask, Stage st, Epi epi) {
+    __shared__ T sU[HN * 3];
+    __shared__ T sW[TWO ? HN * 3 : 1];
+    __shared__ T sk[EN], sm[EN];
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, k0 = blockIdx.z * TZ;
+    const int tid = threadIdx.x;
+    for (int v = tid; v < HN * 3; v += NT) {
+        const int node = v / 3, c = v - 3 * node;
+        const int hx = node % HX, hy = (node / HX) % HY, hz = node / (HX * HY);
+        const int gi = i0 - 1 + hx, gj = j0 - 1 + hy, gk = k0 - 1 + hz;
+        T val = 0, valw = 0;
+        if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) {
+            const long n = gi + (long)g.nx * (gj + (long)g.ny * gk);
+            const bool clamped = MASKIN && ((mask[n] >> c) & 1);
+            if (!clamped) {
+                val = st.k(3 * n + c);
+                if (TWO) valw = st.m(3 * n + c);
+            }
+        }
+        sU[v] = val;
+        if (TWO) sW[v] = valw;
+    }
+    for (int e = tid; e < EN; e += NT) {
+        const int hx = e % EXs, hy = (e / EXs) % EYs, hz = e / (EXs * EYs);
+        const int gi = i0 - 1 + hx, gj = j0 - 1 + hy, gk = k0 - 1 + hz;
+        T a = 0, b = 0;
+        if (gi >= 0 && gi < g.ex && gj >= 0 && gj < g.ey && gk >= 0 && gk < g.ez) {
+            const long ge = gi + (long)g.ex * (gj + (long)g.ey * gk);
+            a = cK * (T)coef_ld(kf, ge);
+            b = cM * (T)coef_ld(mf, ge);
+        }
+        sk[e] = a;
+        sm[e] = b;
+    }
+    __syncthreads();
+    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+    const int gi = i0 + tx, gj = j0 + ty, gk = k0 + tz;
+    const bool inside = gi < g.nx && gj < g.ny && gk < g.nz;
+    T yK[3] = {0, 0, 0}, yM[3] = {0, 0, 0};
+    long n = 0;
+    uint8_t cl = 0;
+    if (inside) {
+        node_contract<T, TWO>(sU, sW, sk, sm, KE, M8, tx, ty, tz, yK, yM);
+        n = gi + (long)g.nx * (gj + (long)g.ny * gk);
+        cl = mask[n];
+    }
+    epi(inside, n, cl, yK, yM, st);
+}
+
+// ---- epilogues.  All are called by every thread (inside or not) so that reducing epilogues can sync.
+template <typename T>
+struct EpiBC {  // y = A u on free dofs, u on clamped
+    T* y;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+        if (!in) return;
+#pragma unroll
+        for (int c = 0; c < 3; c++) y[3 * n + c] = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
+    }
+};
+
+template <typename T>
+struct EpiBCDot {  // y = A u (BC form) and sum u.y -> scal
+    T* y;
+    double* partials;
+    unsigned* counter;
+    double* out;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+        double acc[1] = {0.0};
+        if (in) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const T u = st.k(3 * n + c);
+                const T v = ((cl >> c) & 1) ? u : yK[c] + yM[c];
+                y[3 * n + c] = v;
+                acc[0] += (double)u * (double)v;
+            }
+        }
+        reduce_finish<1>(acc, partials, counter, FinStore1{out});
+    }
+};
+
+template <typename T>
+struct EpiResid {  // r = b - A x
+    const T* b;
+    T* r;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+        if (!in) return;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T ax = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
+            r[3 * n + c] = b[3 * n + c] - ax;
+        }
+    }
+};
+
+template <typename T>
+struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
+    const T* b;
+    const T* d;
+    const T* dinv;
+    T c0, c1;
+    T* xn;
+    T* dn;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+        if (!in) return;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long i = 3 * n + c;
+            const T x = st.k(i);
+            const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
+            const T r = b[i] - ax;
+            T dd = c1 * dinv[i] * r;
+            if (d) dd = fma(c0, d[i], dd);
+            xn[i] = x + dd;
+            if (dn) dn[i] = dd;
+        }
+    }
+};
+
+struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
+    double* yk;
+    double* ym;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const double (&yK)[3], const double (&yM)[3],
+                               const S&) const {
+        if (!in) return;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const bool clamped = (cl >> c) & 1;
+            yk[3 * n + c] = clamped ? 0.0 : yK[c];
+            ym[3 * n + c] = clamped ? 0.0 : yM[c];
+        }
+    }
+};
+
+struct EpiRaw {  // y = yK + yM, no BC
+    double* y;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t, const double (&yK)[3], const double (&yM)[3],
+                               const S&) const {
+        if (!in) return;
+#pragma unroll
+        for (int c = 0; c < 3; c++) y[3 * n + c] = yK[c] + yM[c];
+    }
+};
+
+struct EpiRHS {  // R = N.(fs F + yK + yM)
+    const double* F;
+    double fs;
+    double* R;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const double (&yK)[3], const double (&yM)[3],
+                               const S&) const {
+        if (!in) return;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const long i = 3 * n + c;
+            double r = fs * F[i];
+            r += yK[c];
+            r += yM[c];
+            R[i] = ((cl >> c) & 1) ? 0.0 : r;
+        }
+    }
+};
+
+struct FinAlpha {  // pq -> alpha = gamma / pq
+    double* scal;
+    __device__ void operator()(const double (&t)[1]) const {
+        scal[SC_PQ] = t[0];
+        scal[SC_ALPHA] = scal[SC_GAMMA] / t[0];
+    }
+};
+
+struct EpiFcg {  // p_new = staged p (owned node), q = A p, partial p.q
+    double* pnew;
+    double* q;
+    double* partials;
+    unsigned* counter;
+    double* scal;
+    template <class S>
+    __device__ void operator()(bool in, long n, uint8_t cl, const double (&yK)[3], const double (&yM)[3],
+                               const S& st) const {
+        double acc[1] = {0.0};
+        if (in) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const long i = 3 * n + c;
+                const double p = st.k(i);
+                const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
+                pnew[i] = p;
+                q[i] = v;
+                acc[0] += p * v;
+            }
+        }
+        reduce_finish<1>(acc, partials, counter, FinAlpha{scal});
+    }
+};
+
+static inline dim3 apply_grid(const Grid& g) {
+    return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + TZ - 1) / TZ);
+}
+
+// dispatch on coefficient kind and precision
+template <typename T, bool TWO, bool MASKIN, class Stage, class Epi>
+static void run_apply(const OpLevel& L, double cK, double cM, const Stage& st, const Epi& epi, hipStream_t s) {
+    const T* KE = std::is_same<T, double>::value ? (const T*)L.KE64 : (const T*)L.KE32;
+    const T* M8 = std::is_same<T, double>::value ? (const T*)L.M864 : (const T*)L.M832;
+    dim3 grid = apply_grid(L.g);
+    GridArgs ga = gargs(L.g);
+    switch (L.coef_kind) {
+    case COEF_F64:
+        hipLaunchKernelGGL((k_apply<T, double, TWO, MASKIN, Stage, Epi>), grid, dim3(NT), 0, s, ga, KE, M8,
+                           (const double*)L.kf, (const double*)L.mf, (T)cK, (T)cM, L.mask, st, epi);
+        break;
+    case COEF_F32:
+        hipLaunchKernelGGL((k_apply<T, float, TWO, MASKIN, Stage, Epi>), grid, dim3(NT), 0, s, ga, KE, M8,
+                           (const float*)L.kf, (const float*)L.mf, (T)cK, (T)cM, L.mask, st, epi);
+        break;
+    default:
+        hipLaunchKernelGGL((k_apply<T, bf16_t, TWO, MASKIN, Stage, Epi>), grid, dim3(NT), 0, s, ga, KE, M8,
+                           (const bf16_t*)L.kf, (const bf16_t*)L.mf, (T)cK, (T)cM, L.mask, st, epi);
+        break;
+    }
+    TT_HIP(hipGetLastError());
+}
+
+static inline int apply_nblocks(const Grid& g) {
+    dim3 d = apply_grid(g);
+    return (int)(d.x * d.y * d.z);
+}
+
+void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void* y, bool fp64, hipStream_t s,
+                     Reducer* red, int dot_slot) {
+    if (fp64) {
+        StageOne<double> st{(const double*)u};
+        if (red) {
+            TT_CHECK(apply_nblocks(L.g) <= red->maxblocks, -3, "reducer too small");
+            EpiBCDot<double> e{(double*)y, red->partials.p, red->counter.p, red->scal.p + dot_slot};
+            run_apply<double, false, true>(L, cK, cM, st, e, s);
+        } else {
+            run_apply<double, false, true>(L, cK, cM, st, EpiBC<double>{(double*)y}, s);
+        }
+    } else {
+        StageOne<float> st{(const float*)u};
+        if (red) {
+            TT_CHECK(apply_nblocks(L.g) <= red->maxblocks, -3, "reducer too small");
+            EpiBCDot<float> e{(float*)y, red->partials.p, red->counter.p, red->scal.p + dot_slot};
+            run_apply<float, false, true>(L, cK, cM, st, e, s);
+        } else {
+            run_apply<float, false, true>(L, cK, cM, st, EpiBC<float>{(float*)y}, s);
+        }
+    }
+}
+
+void launch_residual(const OpLevel& L, double cK, double cM, const void* b, const void* x, void* r, bool fp64,
+                     hipStream_t s) {
+    if (fp64)
+        run_apply<double, false, true>(L, cK, cM, StageOne<double>{(const double*)x},
+                                       EpiResid<double>{(const double*)b, (double*)r}, s);
+    else
+        run_apply<float, false, true>(L, cK, cM, StageOne<float>{(const float*)x},
+                                      EpiResid<float>{(const float*)b, (float*)r}, s);
+}
+
+void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
+                      const void* dinv, double c0, double c1, void* xn, void* dn, bool fp64, hipStream_t s) {
+    if (fp64) {
+        EpiCheb<double> e{(const double*)b, (const double*)d, (const double*)dinv, c0, c1, (double*)xn, (double*)dn};
+        run_apply<double, false, true>(L, cK, cM, StageOne<double>{(const double*)x}, e, s);
+    } else {
+        EpiCheb<float> e{(const float*)b,  (const float*)d, (const float*)dinv, (float)c0,
+                         (float)c1,        (float*)xn,      (float*)dn};
+        run_apply<float, false, true>(L, cK, cM, StageOne<float>{(const float*)x}, e, s);
+    }
+}
+
+void launch_apply_split(const OpLevel& L, double cK, double cM, const double* u, double* yk, double* ym,
+                        hipStream_t s) {
+    run_apply<double, false, true>(L, cK, cM, StageOne<double>{u}, EpiSplit{yk, ym}, s);
+}
+
+void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, double cK, double cM, const double* u,
+                      double* y, hipStream_t s) {
+    OpLevel L2 = L;
+    L2.kf = kf;
+    L2.mf = mf;
+    L2.coef_kind = COEF_F64;
+    run_apply<double, false, false>(L2, cK, cM, StageOne<double>{u}, EpiRaw{y}, s);
+}
+
+void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, const double* a, const double* F,
+                        double fs, const double wK[3], const double wM[3], double* R, hipStream_t s) {
+    StageThree st{u, v, a, wK[0], wK[1], wK[2], wM[0], wM[1], wM[2]};
+    run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R}, s);
+}
+
+void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
+                      double* q, Reducer& red, bool first, hipStream_t s) {
+    TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
+    StageFcg<double> st{z, p_old, first ? nullptr : red.scal.p + SC_BETA};
+    EpiFcg e{p_new, q, red.partials.p, red.counter.p, red.scal.p};
+    run_apply<double, false, true>(L, cK, cM, st, e, s);
+}
+
+// ------------------------------------------------------------------ diagonal
+template <typename T, typename CT>
+__global__ void k_diag_inv(GridArgs g, const double* __restrict__ KE, const double* __restrict__ M8,
+                           const CT* __restrict__ kf, const CT* __restrict__ mf, double cK, double cM,
+                           const uint8_t* __restrict__ mask, T* __restrict__ dinv) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)g.nx * g.ny * g.nz) return;
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+    double d[3] = {0, 0, 0};
+    for (int ez = 0; ez < 2; ez++)
+        for (int ey = 0; ey < 2; ey++)
+            for (int ex = 0; ex < 2; ex++) {
+                const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+                if (ei < 0 || ej < 0 || ek < 0 || ei >= g.ex || ej >= g.ey || ek >= g.ez) continue;
+                const long e = ei + (long)g.ex * (ej + (long)g.ey * ek);
+                const int a = corner_id(1 - ex, 1 - ey, 1 - ez);
+                const double kk = cK * coef_ld(kf, e), mm = cM * coef_ld(mf, e);
+                for (int c = 0; c < 3; c++) d[c] += kk * KE[(3 * a + c) * 25] + mm * M8[a * 9];
+            }
+    const uint8_t cl = mask[n];
+    for (int c = 0; c < 3; c++) dinv[3 * n + c] = (T)(((cl >> c) & 1) ? 1.0 : 1.0 / d[c]);
+}
+
+void launch_diag_inv(const OpLevel& L, double cK, double cM, void* dinv, bool fp64, hipStream_t s) {
+    GridArgs ga = gargs(L.g);
+    int nb = nblocks(L.g.nn);
+#define TT_DIAG(T, CT) \
+    hipLaunchKernelGGL((k_diag_inv<T, CT>), dim3(nb), dim3(BS), 0, s, ga, L.KE64, L.M864, (const CT*)L.kf, (const CT*)L.mf, cK, cM, L.mask, (T*)dinv)
+    if (fp64) {
+        if (L.coef_kind == COEF_F64) TT_DIAG(double, double);
+        else if (L.coef_kind == COEF_F32) TT_DIAG(double, float);
+        else TT_DIAG(double, bf16_t);
+    } else {
+        if (L.coef_kind == COEF_F64) TT_DIAG(float, double);
+        else if (L.coef_kind == COEF_F32) TT_DIAG(float, float);
+        else TT_DIAG(float, bf16_t);
+    }
+#undef TT_DIAG
+    TT_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ vector kernels
+template <typename T>
+__global__ void k_fill(T* x, T v, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s) {
+    if (n <= 0) return;
+    if (fp64) hipLaunchKernelGGL(k_fill<double>, dim3(nblocks(n)), dim3(BS), 0, s, (double*)x, v, n);
+    else hipLaunchKernelGGL(k_fill<float>, dim3(nblocks(n)), dim3(BS), 0, s, (float*)x, (float)v, n);
+    TT_HIP(hipGetLastError());
+}
+void launch_fill_scal(void* x, long n, double v, hipStream_t s) { launch_fill(x, v, n, true, s); }
+
+template <typename S, typename D>
+__global__ void k_copy(const S* __restrict__ a, D* __restrict__ b, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (D)a[i];
+}
+void launch_copy(const void* src, void* dst, long n, bool s64, bool d64, hipStream_t s) {
+    if (n <= 0) return;
+    dim3 gr(nblocks(n)), bl(BS);
+    if (s64 && d64) hipLaunchKernelGGL((k_copy<double, double>), gr, bl, 0, s, (const double*)src, (double*)dst, n);
+    else if (s64) hipLaunchKernelGGL((k_copy<double, float>), gr, bl, 0, s, (const double*)src, (float*)dst, n);
+    else if (d64) hipLaunchKernelGGL((k_copy<float, double>), gr, bl, 0, s, (const float*)src, (double*)dst, n);
+    else hipLaunchKernelGGL((k_copy<float, float>), gr, bl, 0, s, (const float*)src, (float*)dst, n);
+    TT_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_cheb_first(const T* __restrict__ b, const T* __restrict__ dinv, T c1, T* __restrict__ x, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) x[i] = c1 * dinv[i] * b[i];
+}
+void launch_cheb_first(const void* b, const void* dinv, double c1, void* x, long n, bool fp64, hipStream_t s) {
+    if (fp64)
+        hipLaunchKernelGGL(k_cheb_first<double>, dim3(nblocks(n)), dim3(BS), 0, s, (const double*)b,
+                           (const double*)dinv, c1, (double*)x, n);
+    else
+        hipLaunchKernelGGL(k_cheb_first<float>, dim3(nblocks(n)), dim3(BS), 0, s, (const float*)b,
+                           (const float*)dinv, (float)c1, (float*)x, n);
+    TT_HIP(hipGetLastError());
+}
+
+// restriction R = P^T of trilinear interpolation (PETSc DMDA Q1 interpolation, LinearElasticity.cc:2840-2844)
+template <typename T>
+__global__ void k_restrict(GridArgs f, GridArgs c, const T* __restrict__ rf, const uint8_t* __restrict__ cmask,
+                           T* __restrict__ bc) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)c.nx * c.ny * c.nz) return;
+    const int I = n % c.nx, J = (n / c.nx) % c.ny, K = n / ((long)c.nx * c.ny);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int dk = -1; dk <= 1; dk++) {
+        const int k = 2 * K + dk;
+        if (k < 0 || k >= f.nz) continue;
+        const T wk = dk == 0 ? (T)1 : (T)0.5;
+        for (int dj = -1; dj <= 1; dj++) {
+            const int j = 2 * J + dj;
+            if (j < 0 || j >= f.ny) continue;
+            const T wj = dj == 0 ? wk : wk * (T)0.5;
+            for (int di = -1; di <= 1; di++) {
+                const int i = 2 * I + di;
+                if (i < 0 || i >= f.nx) continue;
+                const T w = di == 0 ? wj : wj * (T)0.5;
+                const long m = i + (long)f.nx * (j + (long)f.ny * k);
+                s0 = fma(w, rf[3 * m], s0);
+                s1 = fma(w, rf[3 * m + 1], s1);
+                s2 = fma(w, rf[3 * m + 2], s2);
+            }
+        }
+    }
+    const uint8_t cl = cmask[n];
+    bc[3 * n] = (cl & 1) ? (T)0 : s0;
+    bc[3 * n + 1] = (cl & 2) ? (T)0 : s1;
+    bc[3 * n + 2] = (cl & 4) ? (T)0 : s2;
+}
+void launch_restrict(const Grid& fine, const Grid& coarse, const void* rf, const uint8_t* cmask, void* bc, bool fp64,
+                     hipStream_t s) {
+    if (fp64)
+        hipLaunchKernelGGL(k_restrict<double>, dim3(nblocks(coarse.nn)), dim3(BS), 0, s, gargs(fine), gargs(coarse),
+                           (const double*)rf, cmask, (double*)bc);
+    else
+        hipLaunchKernelGGL(k_restrict<float>, dim3(nblocks(coarse.nn)), dim3(BS), 0, s, gargs(fine), gargs(coarse),
+                           (const float*)rf, cmask, (float*)bc);
+    TT_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_prolong_add(GridArgs f, GridArgs c, const T* __restrict__ xc, const uint8_t* __restrict__ fmask,
+                              T* __restrict__ xf) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)f.nx * f.ny * f.nz) return;
+    const int i = n % f.nx, j = (n / f.nx) % f.ny, k = n / ((long)f.nx * f.ny);
+    const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+    const int I1 = I0 + (i & 1), J1 = J0 + (j & 1), K1 = K0 + (k & 1);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int kk = 0; kk < 2; kk++) {
+        if (kk == 1 && K1 == K0) break;
+        const int K = kk ? K1 : K0;
+        const T wk = (K1 == K0) ? (T)1 : (T)0.5;
+        for (int jj = 0; jj < 2; jj++) {
+            if (jj == 1 && J1 == J0) break;
+            const int J = jj ? J1 : J0;
+            const T wj = (J1 == J0) ? wk : wk * (T)0.5;
+            for (int ii = 0; ii < 2; ii++) {
+                if (ii == 1 && I1 == I0) break;
+                const int I = ii ? I1 : I0;
+                const T w = (I1 == I0) ? wj : wj * (T)0.5;
+                const long m = I + (long)c.nx * (J + (long)c.ny * K);
+                s0 = fma(w, xc[3 * m], s0);
+                s1 = fma(w, xc[3 * m + 1], s1);
+                s2 = fma(w, xc[3 * m + 2], s2);
+            }
+        }
+    }
+    const uint8_t cl = fmask[n];
+    if (!(cl & 1)) xf[3 * n] += s0;
+    if (!(cl & 2)) xf[3 * n + 1] += s1;
+    if (!(cl & 4)) xf[3 * n + 2] += s2;
+}
+void launch_prolong_add(const Grid& fine, const Grid& coarse, const void* xc, const uint8_t* fmask, void* xf,
+                        bool fp64, hipStream_t s) {
+    if (fp64)
+        hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblocks(fine.nn)), dim3(BS), 0, s, gargs(fine),
+                           gargs(coarse), (const double*)xc, fmask, (double*)xf);
+    else
+        hipLaunchKernelGGL(k_prolong_add<float>, dim3(nblocks(fine.nn)), dim3(BS), 0, s, gargs(fine), gargs(coarse),
+                           (const float*)xc, fmask, (float*)xf);
+    TT_HIP(hipGetLastError());
+}
+
+// rediscretised coarse coefficients: mean of the 8 child elements
+__global__ void k_coarsen_coef(GridArgs f, GridArgs c, const double* __restrict__ kff, const double* __restrict__ mff,
+                               double* __restrict__ kfc, double* __restrict__ mfc) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)c.ex * c.ey * c.ez) return;
+    const int I = e % c.ex, J = (e / c.ex) % c.ey, K = e / ((long)c.ex * c.ey);
+    double sk = 0.0, sm = 0.0;
+    for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) {
+                const long fe = (2 * I + di) + (long)f.ex * ((2 * J + dj) + (long)f.ey * (2 * K + dk));
+                sk += kff[fe];
+                sm += mff[fe];
+            }
+    kfc[e] = 0.125 * sk;
+    mfc[e] = 0.125 * sm;
+}
+void launch_coarsen_coef(const Grid& fine, const Grid& coarse, const double* kf_f, const double* mf_f, double* kf_c,
+                         double* mf_c, hipStream_t s) {
+    hipLaunchKernelGGL(k_coarsen_coef, dim3(nblocks(coarse.nel)), dim3(BS), 0, s, gargs(fine), gargs(coarse), kf_f,
+                       mf_f, kf_c, mf_c);
+    TT_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ bf16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16_t)(u >> 16);
+}
+__global__ void k_convert_coef(const double* __restrict__ src, void* dst, long n, int kind) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (kind == COEF_F32) ((float*)dst)[i] = (float)src[i];
+    else if (kind == COEF_BF16) ((bf16_t*)dst)[i] = f2bf((float)src[i]);
+    else ((double*)dst)[i] = src[i];
+}
+void launch_convert_coef(const double* src, void* dst, long n, int kind, hipStream_t s) {
+    hipLaunchKernelGGL(k_convert_coef, dim3(nblocks(n)), dim3(BS), 0, s, src, dst, n, kind);
+    TT_HIP(hipGetLastError());
+}
+
+// SIMP / RAMP / linear interpolation (LinearElasticity.cc:1580-1623)
+__device__ __forceinline__ void interp1(int scheme, double vmin, double vmax, double p, double x, double& f,
+                                        double& df) {
+    if (scheme == 0) {
+        f = vmin + x * (vmax - vmin);
+        df = vmax - vmin;
+    } else if (scheme == 1) {
+        f = vmin + pow(x, p) * (vmax - vmin);
+        df = p * pow(x, p - 1) * (vmax - vmin);
+    } else {
+        f = vmin + x / (1 + p * (1 - x)) * (vmax - vmin);
+        df = (1 + p) / pow(1 + p * (1 - x), 2) * (vmax - vmin);
+    }
+}
+__global__ void k_interp(const double* __restrict__ x, long n, int sInt, double Emin, double Emax, double pK, int mInt,
+                         double Rmin, double Rmax, double pM, const double* __restrict__ P, double* kf, double* mf,
+                         double* dkf, double* dmf, double* Pkf, double* Pmf) {
+    long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double k, dk, m, dm;
+    interp1(sInt, Emin, Emax, pK, x[e], k, dk);
+    interp1(mInt, Rmin, Rmax, pM, x[e], m, dm);
+    kf[e] = k;
+    mf[e] = m;
+    dkf[e] = dk;
+    dmf[e] = dm;
+    Pkf[e] = P[e] * k;
+    Pmf[e] = P[e] * m;
+}
+void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, double pK, int mInt, double Rmin,
+                   double Rmax, double pM, const double* P, double* kf, double* mf, double* dkf, double* dmf,
+                   double* Pkf, double* Pmf, hipStream_t s) {
+    hipLaunchKernelGGL(k_interp, dim3(nblocks(n)), dim3(BS), 0, s, x, n, sInt, Emin, Emax, pK, mInt, Rmin, Rmax, pM,
+                       P, kf, mf, dkf, dmf, Pkf, Pmf);
+    TT_HIP(hipGetLastError());
+}
+
+// dense coarsest-level solve x = Ainv b (Ainv column-major, n x n)
+__global__ void k_dense_gemv(const double* __restrict__ A, const double* __restrict__ b, double* __restrict__ x,
+                             int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; j++) s = fma(A[(size_t)j * n + i], b[j], s);
+    x[i] = s;
+}
+void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_dense_gemv, dim3(nblocks(n, 64)), dim3(64), 0, s, Ainv, b, x, n);
+    TT_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ reductions / Krylov
+__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n, double* partials,
+                      unsigned* counter, double* out) {
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        acc[0] = fma(a[i], b[i], acc[0]);
+    reduce_finish<1>(acc, partials, counter, FinStore1{out});
+}
+void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(BS), 0, s, a, b, n, red.partials.p, red.counter.p,
+                       red.scal.p + slot);
+    TT_HIP(hipGetLastError());
+}
+
+struct FinFcgDots {  // gamma = z.r, delta = z.r_prev; beta = (gamma - delta)/gamma_old
+    double* scal;
+    bool first;
+    __device__ void operator()(const double (&t)[2]) const {
+        scal[SC_BETA] = first ? 0.0 : (t[0] - t[1]) / scal[SC_GAMMA_OLD];
+        scal[SC_GAMMA] = t[0];
+        scal[SC_DELTA] = t[1];
+        scal[SC_GAMMA_OLD] = t[0];
+    }
+};
+__global__ void k_fcg_dots(const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ rp,
+                           long n, double* partials, unsigned* counter, double* scal, bool first) {
+    double acc[2] = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        acc[0] = fma(z[i], r[i], acc[0]);
+        if (!first) acc[1] = fma(z[i], rp[i], acc[1]);
+    }
+    reduce_finish<2>(acc, partials, counter, FinFcgDots{scal, first});
+}
+void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_dots, dim3(red_blocks(n)), dim3(BS), 0, s, z, r, rprev, n, red.partials.p,
+                       red.counter.p, red.scal.p, first);
+    TT_HIP(hipGetLastError());
+}
+
+__global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ p, double* __restrict__ r,
+                             double* __restrict__ rp, const double* __restrict__ q, long n, double* partials,
+                             unsigned* counter, double* scal) {
+    const double al = scal[SC_ALPHA];
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        x[i] = fma(al, p[i], x[i]);
+        const double ri = r[i];
+        rp[i] = ri;
+        const double rn = fma(-al, q[i], ri);
+        r[i] = rn;
+        acc[0] = fma(rn, rn, acc[0]);
+    }
+    reduce_finish<1>(acc, partials, counter, FinStore1{scal + SC_RR});
+}
+void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_update, dim3(red_blocks(n)), dim3(BS), 0, s, x, p, r, rprev, q, n, red.partials.p,
+                       red.counter.p, red.scal.p);
+    TT_HIP(hipGetLastError());
+}
+
+__global__ void k_jacobi(const double* __restrict__ dinv, const double* __restrict__ r, double* __restrict__ z,
+                         long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) z[i] = dinv[i] * r[i];
+}
+void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi, dim3(nblocks(n)), dim3(BS), 0, s, dinv, r, z, n);
+    TT_HIP(hipGetLastError());
+}
+
+// power iteration helper: x = dinv.Ax, and ||x||^2 -> scal[SC_TMP0]
+template <typename T>
+__global__ void k_norm_power(const T* __restrict__ Ax, const T* __restrict__ dinv, T* __restrict__ x, long n,
+                             double* partials, unsigned* counter, double* out) {
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const T v = dinv[i] * Ax[i];
+        x[i] = v;
+        acc[0] += (double)v * (double)v;
+    }
+    reduce_finish<1>(acc, partials, counter, FinStore1{out});
+}
+void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s) {
+    if (fp64)
+        hipLaunchKernelGGL(k_norm_power<double>, dim3(red_blocks(n)), dim3(BS), 0, s, (const double*)Ax,
+                           (const double*)dinv, (double*)x, n, red.partials.p, red.counter.p, red.scal.p + SC_TMP0);
+    else
+        hipLaunchKernelGGL(k_norm_power<float>, dim3(red_blocks(n)), dim3(BS), 0, s, (const float*)Ax,
+                           (const float*)dinv, (float*)x, n, red.partials.p, red.counter.p, red.scal.p + SC_TMP0);
+    TT_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_scale_by_scal(T* x, long n, const double* scal, int slot) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (T)((double)x[i] / sqrt(scal[slot]));
+}
+void launch_scale_by_scal(void* x, long n, const double* scal, int slot, bool fp64, hipStream_t s) {
+    if (fp64) hipLaunchKernelGGL(k_scale_by_scal<double>, dim3(nblocks(n)), dim3(BS), 0, s, (double*)x, n, scal, slot);
+    else hipLaunchKernelGGL(k_scale_by_scal<float>, dim3(nblocks(n)), dim3(BS), 0, s, (float*)x, n, scal, slot);
+    TT_HIP(hipGetLastError());
+}
+
+__global__ void k_scale(double* x, long n, double a) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+void launch_scale(double* x, long n, double a, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale, dim3(nblocks(n)), dim3(BS), 0, s, x, n, a);
+    TT_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ Newmark / adjoint vector kernels
+// corrector (LinearElasticity.cc:1363-1375): c = a[0..5]
+__global__ void k_newmark_update(const double* __restrict__ u, const double* __restrict__ v,
+                                 const double* __restrict__ a, const double* __restrict__ un, double* __restrict__ vn,
+                                 double* __restrict__ an, double a0, double a1, double a2, double a3, double a4,
+                                 double a5, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double du = un[i] - u[i];
+    double vv = 0.0;
+    vv += a0 * v[i];
+    vv += a1 * a[i];
+    vv += a2 * du;
+    double aa = 0.0;
+    aa += -a3 * v[i];
+    aa += -a4 * a[i];
+    aa += a5 * du;
+    vn[i] = vv;
+    an[i] = aa;
+}
+void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,
+                           double* an, const double* c, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_newmark_update, dim3(nblocks(n)), dim3(BS), 0, s, u, v, a, un, vn, an, c[0], c[1], c[2],
+                       c[3], c[4], c[5], n);
+    TT_HIP(hipGetLastError());
+}
+
+// Adjoint step combination (LinearElasticity.cc:864-925).  coef = {a0..a5, dalpha, dbeta}.
+// Kl = N K N lambda_{t+1}, Ml = N M N lambda_{t+1}; B1 = a5 M + a2 C, B2 = a3 M - a0 C, B3 = a4 M - a1 C.
+// rhs = N.(-R1): the SPD form of the reference's solve with A = -(K + a2 C + a5 M).
+struct AdjCoef {
+    double a[6], al, be;
+};
+__global__ void k_adjoint_combine(const double* __restrict__ Kl, const double* __restrict__ Ml,
+                                  const double* __restrict__ hdu, const double* __restrict__ hddu,
+                                  const double* __restrict__ hdddu, const double* __restrict__ mun,
+                                  const double* __restrict__ nun, const uint8_t* __restrict__ mask, AdjCoef c,
+                                  int terminal, double* __restrict__ mu, double* __restrict__ nu,
+                                  double* __restrict__ rhs, long nd) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= nd) return;
+    const double* a = c.a;
+    double m, v, r;
+    if (terminal) {
+        m = -hddu[i];
+        v = -hdddu[i];
+        r = hdu[i];
+        r += -a[2] * m;
+        r += -a[5] * v;
+        r = -r;
+    } else {
+        const double kl = Kl[i], ml = Ml[i];
+        const double b1 = (a[5] + a[2] * c.al) * ml + a[2] * c.be * kl;
+        const double b2 = (a[3] - a[0] * c.al) * ml - a[0] * c.be * kl;
+        const double b3 = (a[4] - a[1] * c.al) * ml - a[1] * c.be * kl;
+        double mm = hddu[i];
+        mm += b2;
+        mm += -a[0] * mun[i];
+        mm += a[3] * nun[i];
+        double vv = hdddu[i];
+        vv += b3;
+        vv += a[4] * nun[i];
+        r = hdu[i];
+        r += b1;
+        r += a[2] * mun[i];
+        r += a[5] * nun[i];
+        r = -r;
+        m = -mm;
+        v = -vv;
+        r += a[2] * m;
+        r += a[5] * v;
+    }
+    mu[i] = m;
+    nu[i] = v;
+    const bool clamped = (mask[i / 3] >> (i % 3)) & 1;
+    rhs[i] = clamped ? 0.0 : -r;
+}
+void launch_adjoint_combine(const double* Kl, const double* Ml, const double* hdu, const double* hddu,
+                            const double* hdddu, const double* mun, const double* nun, const uint8_t* mask,
+                            const double* coef, bool terminal, double* mu, double* nu, double* rhs, long nn,
+                            hipStream_t s) {
+    AdjCoef c;
+    for (int k = 0; k < 6; k++) c.a[k] = coef[k];
+    c.al = coef[6];
+    c.be = coef[7];
+    long nd = 3 * nn;
+    hipLaunchKernelGGL(k_adjoint_combine, dim3(nblocks(nd)), dim3(BS), 0, s, Kl, Ml, hdu, hddu, hdddu, mun, nun, mask,
+                       c, (int)terminal, mu, nu, rhs, nd);
+    TT_HIP(hipGetLastError());
+}
+
+// t = 0 adjoint right-hand side (LinearElasticity.cc:753-776): R3 = N.-(HDDDU + B3 l1 - a1 mu1 + a4 nu1)
+__global__ void k_adjoint0_rhs(const double* __restrict__ Kl, const double* __restrict__ Ml,
+                               const double* __restrict__ hdddu, const double* __restrict__ mun,
+                               const double* __restrict__ nun, const uint8_t* __restrict__ mask, AdjCoef c,
+                               double* __restrict__ rhs, long nd) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= nd) return;
+    const double* a = c.a;
+    const double b3 = (a[4] - a[1] * c.al) * Ml[i] - a[1] * c.be * Kl[i];
+    double r = hdddu[i];
+    r += b3;
+    r += -a[1] * mun[i];
+    r += a[4] * nun[i];
+    const bool clamped = (mask[i / 3] >> (i % 3)) & 1;
+    rhs[i] = clamped ? 0.0 : -r;
+}
+void launch_adjoint0_rhs(const double* Kl, const double* Ml, const double* hdddu, const double* mun,
+                         const double* nun, const uint8_t* mask, const double* coef, double* rhs, long nn,
+                         hipStream_t s) {
+    AdjCoef c;
+    for (int k = 0; k < 6; k++) c.a[k] = coef[k];
+    c.al = coef[6];
+    c.be = coef[7];
+    long nd = 3 * nn;
+    hipLaunchKernelGGL(k_adjoint0_rhs, dim3(nblocks(nd)), dim3(BS), 0, s, Kl, Ml, hdddu, mun, nun, mask, c, rhs, nd);
+    TT_HIP(hipGetLastError());
+}
+
+// objectives 0..2: d/du of sum_e P_e dt/2 sum_k w_k^2 = dt * w_d * (sum of P over elements at the node)
+__global__ void k_hdiag(const double* __restrict__ w, const double* __restrict__ cnt, double dt, double* __restrict__ h,
+                        long nd) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < nd) h[i] = dt * w[i] * cnt[i / 3];
+}
+void launch_hdiag(const double* w, const double* cnt, double dt, double* h, long nn, hipStream_t s) {
+    hipLaunchKernelGGL(k_hdiag, dim3(nblocks(3 * nn)), dim3(BS), 0, s, w, cnt, dt, h, 3 * nn);
+    TT_HIP(hipGetLastError());
+}
+__global__ void k_region_count(GridArgs g, const double* __restrict__ P, double* __restrict__ cnt) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)g.nx * g.ny * g.nz) return;
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+    double s = 0.0;
+    for (int ez = 0; ez < 2; ez++)
+        for (int ey = 0; ey < 2; ey++)
+            for (int ex = 0; ex < 2; ex++) {
+                const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+                if (ei < 0 || ej < 0 || ek < 0 || ei >= g.ex || ej >= g.ey || ek >= g.ez) continue;
+                s += P[ei + (long)g.ex * (ej + (long)g.ey * ek)];
+            }
+    cnt[n] = s;
+}
+void launch_region_count(const Grid& g, const double* P, double* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_region_count, dim3(nblocks(g.nn)), dim3(BS), 0, s, gargs(g), P, cnt);
+    TT_HIP(hipGetLastError());
+}
+
+}  // namespace tt

@@ -1,0 +1,581 @@
+// HIP kernels for gfx950: element sensitivities and objective, PDE (Helmholtz)
+// filter, Heaviside projection, and the element-wise parts of MMA.
+#include "tt_internal.h"
+
+namespace tt {
+
+constexpr int BS2 = 256;
+static inline int nblk(long n, int per = BS2) { return (int)((n + per - 1) / per); }
+static inline int rblk(long n) {
+    int b = nblk(n);
+    return b > 2048 ? 2048 : (b < 1 ? 1 : b);
+}
+
+// ---- deterministic reduction (same protocol as tt_kernels.hip)
+template <int NV>
+__device__ __forceinline__ void bsum(double (&v)[NV], double* sh) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[k] = x;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[wid * NV + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w * NV + k];
+            v[k] = s;
+        }
+    }
+}
+template <int NV>
+__device__ __forceinline__ void rfinish(double (&v)[NV], double* partials, unsigned* counter, double* out) {
+    __shared__ double sh[16 * NV];
+    __shared__ bool last;
+    bsum<NV>(v, sh);
+    const unsigned nb = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[(size_t)blockIdx.x * NV + k] = v[k];
+        __threadfence();
+        last = (atomicAdd(counter, 1u) == nb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    bsum<NV>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] = acc[k];
+        *counter = 0u;
+    }
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+    atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+struct GA {
+    int nx, ny, nz, ex, ey, ez;
+};
+static inline GA ga(const Grid& g) { return GA{g.nx, g.ny, g.nz, g.ex, g.ey, g.ez}; }
+
+__device__ __forceinline__ void elem_nodes(const GA& g, long e, long (&nd)[8]) {
+    const long i = e % g.ex, j = (e / g.ex) % g.ey, k = e / ((long)g.ex * g.ey);
+    const long n0 = i + (long)g.nx * (j + (long)g.ny * k), sy = g.nx, sz = (long)g.nx * g.ny;
+    nd[0] = n0;
+    nd[1] = n0 + 1;
+    nd[2] = n0 + 1 + sy;
+    nd[3] = n0 + sy;
+    nd[4] = n0 + sz;
+    nd[5] = n0 + 1 + sz;
+    nd[6] = n0 + 1 + sy + sz;
+    nd[7] = n0 + sy + sz;
+}
+
+// ------------------------------------------------------------------ sensitivities
+// Per element and time step (ComputeSensitivitiesElementLoopn, LinearElasticity.cc:1911-2077):
+//   df_e += P_e*dfx0 + lambda_e^T (dA u_t + dB1 u_{t-1} + dB2 v_{t-1} + dB3 a_{t-1})
+// which factorises into dkfac * lambda^T KE w_K + dmfac * lambda^T ME w_M with
+// w_K, w_M linear combinations of (u_t, u_{t-1}, v_{t-1}, a_{t-1}) given by wK/wM.
+// mode 1 is the t = 0 loop (ComputeSensitivitiesElementLoop0, :2080-2204) with
+// the slots (u0, v0, a0): w_K = u0 + dbeta v0, w_M = dalpha v0 + a0.
+struct SensDev {
+    GA g;
+    const double* KE;
+    const double* M8;
+    const double *kf, *mf, *dkf, *dmf, *P;
+    const double *lam, *u, *v, *a, *s1, *s2, *s3;
+    double wK[4], wM[4];
+    double dt, al, be;
+    int objective;
+    double* df;
+};
+
+template <bool OBJ3>
+__global__ __launch_bounds__(BS2) void k_sens(SensDev s, double* partials, unsigned* counter, double* out) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long nel = (long)s.g.ex * s.g.ey * s.g.ez;
+    double h[1] = {0.0};
+    if (e < nel) {
+        long nd[8];
+        elem_nodes(s.g, e, nd);
+        double w[24];
+        // w_K
+#pragma unroll
+        for (int q = 0; q < 24; q++) {
+            const long d = 3 * nd[q / 3] + q % 3;
+            double x = s.wK[0] * s.u[d];
+            if (s.wK[1] != 0.0) x = fma(s.wK[1], s.s1[d], x);
+            if (s.wK[2] != 0.0) x = fma(s.wK[2], s.s2[d], x);
+            if (s.wK[3] != 0.0) x = fma(s.wK[3], s.s3[d], x);
+            w[q] = x;
+        }
+        double tK = 0.0;
+#pragma unroll
+        for (int r = 0; r < 24; r++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
+            tK = fma(s.lam[3 * nd[r / 3] + r % 3], acc, tK);
+        }
+        // w_M (mass is M8 (x) I3)
+#pragma unroll
+        for (int q = 0; q < 24; q++) {
+            const long d = 3 * nd[q / 3] + q % 3;
+            double x = s.wM[0] * s.u[d];
+            if (s.wM[1] != 0.0) x = fma(s.wM[1], s.s1[d], x);
+            if (s.wM[2] != 0.0) x = fma(s.wM[2], s.s2[d], x);
+            if (s.wM[3] != 0.0) x = fma(s.wM[3], s.s3[d], x);
+            w[q] = x;
+        }
+        double tM = 0.0;
+#pragma unroll
+        for (int ai = 0; ai < 8; ai++) {
+            double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const double mab = s.M8[ai * 8 + b];
+                m0 = fma(mab, w[3 * b], m0);
+                m1 = fma(mab, w[3 * b + 1], m1);
+                m2 = fma(mab, w[3 * b + 2], m2);
+            }
+            const long d = 3 * nd[ai];
+            tM = fma(s.lam[d], m0, fma(s.lam[d + 1], m1, fma(s.lam[d + 2], m2, tM)));
+        }
+        const double dkf = s.dkf[e], dmf = s.dmf[e], kf = s.kf[e], mf = s.mf[e], P = s.P[e];
+        const double t3 = dkf * tK + dmf * tM;
+        // objective terms of state t (EvaluateObjective / EvaluateObjectiveGradient, :1416-1514)
+        double h0 = 0.0, dfx0 = 0.0;
+        const double hdt = s.dt * 0.5;
+        if (OBJ3) {
+#pragma unroll
+            for (int q = 0; q < 24; q++) w[q] = s.u[3 * nd[q / 3] + q % 3];
+            double uKu = 0.0;
+#pragma unroll
+            for (int r = 0; r < 24; r++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
+                uKu = fma(w[r], acc, uKu);
+            }
+            h0 = hdt * (kf * uKu);
+            dfx0 = hdt * (dkf * uKu);
+        } else {
+            double uKu = 0.0, vKv = 0.0, vMv = 0.0, uu = 0.0, vv = 0.0, aa = 0.0;
+#pragma unroll
+            for (int q = 0; q < 24; q++) w[q] = s.u[3 * nd[q / 3] + q % 3];
+            for (int r = 0; r < 24; r++) {
+                double acc = 0.0;
+                for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
+                uKu = fma(w[r], acc, uKu);
+                uu = fma(w[r], w[r], uu);
+            }
+#pragma unroll
+            for (int q = 0; q < 24; q++) w[q] = s.v[3 * nd[q / 3] + q % 3];
+            for (int r = 0; r < 24; r++) {
+                double acc = 0.0;
+                for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
+                vKv = fma(w[r], acc, vKv);
+                vv = fma(w[r], w[r], vv);
+            }
+            for (int ai = 0; ai < 8; ai++)
+                for (int b = 0; b < 8; b++)
+                    for (int c = 0; c < 3; c++) vMv = fma(w[3 * ai + c] * s.M8[ai * 8 + b], w[3 * b + c], vMv);
+            for (int q = 0; q < 24; q++) {
+                const double x = s.a[3 * nd[q / 3] + q % 3];
+                aa = fma(x, x, aa);
+            }
+            switch (s.objective) {
+            case 0: h0 = hdt * uu; break;
+            case 1: h0 = hdt * vv; break;
+            case 2: h0 = hdt * aa; break;
+            case 3: h0 = hdt * (kf * uKu); dfx0 = hdt * (dkf * uKu); break;
+            case 4: h0 = hdt * (mf * vMv); dfx0 = hdt * (dmf * vMv); break;
+            case 5:
+                h0 = hdt * (kf * uKu + mf * vMv);
+                dfx0 = hdt * (dkf * uKu + dmf * vMv);
+                break;
+            default:
+                h0 = hdt * (s.al * mf * vMv + s.be * kf * vKv);
+                dfx0 = hdt * (s.al * dmf * vMv + s.be * dkf * vKv);
+                break;
+            }
+        }
+        s.df[e] += P * dfx0 + t3;
+        h[0] = h0 * P;
+    }
+    rfinish<1>(h, partials, counter, out);
+}
+
+void launch_sens(const SensArgs& a, Reducer& red, double* fx_out, hipStream_t st) {
+    SensDev s;
+    s.g = ga(a.g);
+    s.KE = a.KE;
+    s.M8 = a.M8;
+    s.kf = a.kf; s.mf = a.mf; s.dkf = a.dkf; s.dmf = a.dmf; s.P = a.P;
+    s.lam = a.lam;
+    s.v = a.v;
+    s.a = a.a;
+    s.dt = a.dt; s.al = a.dalpha; s.be = a.dbeta;
+    s.objective = a.objective;
+    s.df = a.df;
+    if (a.mode == 0) {
+        s.u = a.u; s.s1 = a.ub; s.s2 = a.vb; s.s3 = a.ab;
+    } else {
+        s.u = a.u; s.s1 = a.v; s.s2 = a.a; s.s3 = a.a;
+    }
+    for (int k = 0; k < 4; k++) { s.wK[k] = a.wK[k]; s.wM[k] = a.wM[k]; }
+    const int nb = nblk(a.g.nel);
+    TT_CHECK(nb <= red.maxblocks, -3, "reducer too small for sensitivity kernel");
+    if (a.objective == 3)
+        hipLaunchKernelGGL(k_sens<true>, dim3(nb), dim3(BS2), 0, st, s, red.partials.p, red.counter.p, fx_out);
+    else
+        hipLaunchKernelGGL(k_sens<false>, dim3(nb), dim3(BS2), 0, st, s, red.partials.p, red.counter.p, fx_out);
+    TT_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ PDE filter (PDEFilter.cc)
+// y_n = sum_{e at n} sum_b KF[a][b] U_{node(e,b)}  (constant-coefficient scalar hex8 operator)
+__global__ void k_filter_apply(GA g, const double* __restrict__ KF, const double* __restrict__ u,
+                               double* __restrict__ y, double* partials, unsigned* counter, double* out) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long nn = (long)g.nx * g.ny * g.nz;
+    double acc[1] = {0.0};
+    if (n < nn) {
+        const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+        double s = 0.0;
+#pragma unroll
+        for (int ez = 0; ez < 2; ez++)
+#pragma unroll
+            for (int ey = 0; ey < 2; ey++)
+#pragma unroll
+                for (int ex = 0; ex < 2; ex++) {
+                    const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+                    if (ei < 0 || ej < 0 || ek < 0 || ei >= g.ex || ej >= g.ey || ek >= g.ez) continue;
+                    const int a = (1 - ez) * 4 + ((1 - ey) == 0 ? ((1 - ex) == 0 ? 0 : 1) : ((1 - ex) == 1 ? 2 : 3));
+#pragma unroll
+                    for (int b = 0; b < 8; b++) {
+                        const int bx = ((b & 3) == 1 || (b & 3) == 2), by = (b & 3) >= 2, bz = b >> 2;
+                        const long m = (ei + bx) + (long)g.nx * ((ej + by) + (long)g.ny * (ek + bz));
+                        s = fma(KF[a * 8 + b], u[m], s);
+                    }
+                }
+        y[n] = s;
+        acc[0] = s * u[n];
+    }
+    if (partials) rfinish<1>(acc, partials, counter, out);
+}
+void launch_filter_apply(const Grid& g, const double* kf, const double* u, double* y, Reducer* red, int dot_slot,
+                         hipStream_t s) {
+    const int nb = nblk(g.nn);
+    if (red) {
+        TT_CHECK(nb <= red->maxblocks, -3, "reducer too small for filter apply");
+        hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(BS2), 0, s, ga(g), kf, u, y, red->partials.p,
+                           red->counter.p, red->scal.p + dot_slot);
+    } else {
+        hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(BS2), 0, s, ga(g), kf, u, y, (double*)nullptr,
+                           (unsigned*)nullptr, (double*)nullptr);
+    }
+    TT_HIP(hipGetLastError());
+}
+
+// rhs_n = scale * sum_{e at n} x_e / 8   (T x, PDEFilter.cc:580-588, scaled by the element volume at :208)
+__global__ void k_filter_T(GA g, const double* __restrict__ x, double scale, double* __restrict__ rhs) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)g.nx * g.ny * g.nz) return;
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+    double s = 0.0;
+    for (int ez = 0; ez < 2; ez++)
+        for (int ey = 0; ey < 2; ey++)
+            for (int ex = 0; ex < 2; ex++) {
+                const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+                if (ei < 0 || ej < 0 || ek < 0 || ei >= g.ex || ej >= g.ey || ek >= g.ez) continue;
+                s += 0.125 * x[ei + (long)g.ex * (ej + (long)g.ey * ek)];
+            }
+    rhs[n] = s * scale;
+}
+void launch_filter_T(const Grid& g, const double* x, double scale, double* rhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_filter_T, dim3(nblk(g.nn)), dim3(BS2), 0, s, ga(g), x, scale, rhs);
+    TT_HIP(hipGetLastError());
+}
+// out_e = sum_{n in e} U_n / 8  (T' U, PDEFilter.cc:216)
+__global__ void k_filter_Tt(GA g, const double* __restrict__ U, double* __restrict__ out) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)g.ex * g.ey * g.ez) return;
+    long nd[8];
+    elem_nodes(g, e, nd);
+    double s = 0.0;
+    for (int a = 0; a < 8; a++) s += 0.125 * U[nd[a]];
+    out[e] = s;
+}
+void launch_filter_Tt(const Grid& g, const double* U, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_filter_Tt, dim3(nblk(g.nel)), dim3(BS2), 0, s, ga(g), U, out);
+    TT_HIP(hipGetLastError());
+}
+__global__ void k_filter_diag(GA g, const double* __restrict__ KF, double* __restrict__ dinv) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n >= (long)g.nx * g.ny * g.nz) return;
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+    double s = 0.0;
+    for (int ez = 0; ez < 2; ez++)
+        for (int ey = 0; ey < 2; ey++)
+            for (int ex = 0; ex < 2; ex++) {
+                const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+                if (ei < 0 || ej < 0 || ek < 0 || ei >= g.ex || ej >= g.ey || ek >= g.ez) continue;
+                const int a = (1 - ez) * 4 + ((1 - ey) == 0 ? ((1 - ex) == 0 ? 0 : 1) : ((1 - ex) == 1 ? 2 : 3));
+                s += KF[a * 9];
+            }
+    dinv[n] = 1.0 / s;
+}
+void launch_filter_diag(const Grid& g, const double* kf, double* dinv, hipStream_t s) {
+    hipLaunchKernelGGL(k_filter_diag, dim3(nblk(g.nn)), dim3(BS2), 0, s, ga(g), kf, dinv);
+    TT_HIP(hipGetLastError());
+}
+
+// clamp to [0,1] (Filter.cc:83-102) and Heaviside projection (Filter.h:80-83)
+__global__ void k_clamp_project(const double* __restrict__ in, double* __restrict__ xt, double* __restrict__ xp,
+                                long n, int projection, double beta, double eta) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = in[i];
+    if (x < 0.0) x = 0.0;
+    if (x > 1.0) x = 1.0;
+    xt[i] = x;
+    xp[i] = projection ? (tanh(beta * eta) + tanh(beta * (x - eta))) / (tanh(beta * eta) + tanh(beta * (1.0 - eta)))
+                       : x;
+}
+void launch_clamp_project(const double* xt_in, double* xt, double* xp, long n, int projection, double beta,
+                          double eta, hipStream_t s) {
+    hipLaunchKernelGGL(k_clamp_project, dim3(nblk(n)), dim3(BS2), 0, s, xt_in, xt, xp, n, projection, beta, eta);
+    TT_HIP(hipGetLastError());
+}
+// chain rule of the projection (Filter.h:85-88, Filter.cc:127-160)
+__global__ void k_project_chain(const double* __restrict__ xt, double* __restrict__ df, double* __restrict__ dg,
+                                long n, double beta, double eta) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d =
+        beta * (1.0 - pow(tanh(beta * (xt[i] - eta)), 2.0)) / (tanh(beta * eta) + tanh(beta * (1.0 - eta)));
+    df[i] = df[i] * d;
+    if (dg) dg[i] = dg[i] * d;
+}
+void launch_project_chain(const double* xt, double* df, double* dg, long n, double beta, double eta, hipStream_t s) {
+    hipLaunchKernelGGL(k_project_chain, dim3(nblk(n)), dim3(BS2), 0, s, xt, df, dg, n, beta, eta);
+    TT_HIP(hipGetLastError());
+}
+
+__global__ void k_sum(const double* __restrict__ x, long n, int mnd, double* partials, unsigned* counter,
+                      double* out) {
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        acc[0] += mnd ? 4 * x[i] * (1.0 - x[i]) : x[i];
+    rfinish<1>(acc, partials, counter, out);
+}
+void launch_sum(const double* x, long n, Reducer& red, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 0, red.partials.p, red.counter.p,
+                       red.scal.p + slot);
+    TT_HIP(hipGetLastError());
+}
+void launch_mnd(const double* x, long n, Reducer& red, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 1, red.partials.p, red.counter.p,
+                       red.scal.p + slot);
+    TT_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ MMA (MMA.cc), m = 1
+struct MD {
+    long n;
+    double *L, *U, *alpha, *beta, *p0, *q0, *pij, *qij, *xo1, *xo2;
+};
+static inline MD md(const MMADev& m) { return MD{m.n, m.L, m.U, m.alpha, m.beta, m.p0, m.q0, m.pij, m.qij, m.xo1, m.xo2}; }
+
+// GenSub (MMA.cc:487-600) element part; b partial sum -> out[0]; also xo2 = xo1, xo1 = x (Update, :506-509)
+__global__ void k_mma_gensub(MD m, const double* __restrict__ x, const double* __restrict__ df,
+                             const double* __restrict__ dg, const double* __restrict__ xmin,
+                             const double* __restrict__ xmax, int k, double ainit, double adec, double ainc,
+                             double* partials, unsigned* counter, double* out) {
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m.n; i += (long)gridDim.x * blockDim.x) {
+        const double xv = x[i];
+        double L, U;
+        if (k < 3) {
+            L = xv - ainit * xmax[i];
+            L += ainit * xmin[i];
+            U = xv + ainit * xmax[i];
+            U += -ainit * xmin[i];
+        } else {
+            L = m.L[i];
+            U = m.U[i];
+            const double x1 = m.xo1[i], x2 = m.xo2[i];
+            const double hv = (xv - x1) * (x1 - x2);
+            const double gam = hv < 0.0 ? adec : (hv > 0.0 ? ainc : 1.0);
+            L = xv - gam * (x1 - L);
+            U = xv + gam * (U - x1);
+            const double xmi = fmax(1.0e-5, xmax[i] - xmin[i]);
+            L = fmax(L, xv - 10.0 * xmi);
+            L = fmin(L, xv - 0.01 * xmi);
+            U = fmax(U, xv + 0.01 * xmi);
+            U = fmin(U, xv + 10.0 * xmi);
+        }
+        m.L[i] = L;
+        m.U[i] = U;
+        m.alpha[i] = fmax(xmin[i], 0.9 * L + 0.1 * xv);
+        m.beta[i] = fmin(xmax[i], 0.9 * U + 0.1 * xv);
+        const double feps = 1.0e-6, d = df[i];
+        const double dfp = fmax(0.0, d), dfm = fmax(0.0, -1.0 * d);
+        m.p0[i] = pow(U - xv, 2.0) * (dfp + 0.001 * fabs(d) + 0.5 * feps / (U - L));
+        m.q0[i] = pow(xv - L, 2.0) * (dfm + 0.001 * fabs(d) + 0.5 * feps / (U - L));
+        const double g = dg[i];
+        const double pij = pow(U - xv, 2.0) * fmax(0.0, g);
+        const double qij = pow(xv - L, 2.0) * fmax(0.0, -1.0 * g);
+        m.pij[i] = pij;
+        m.qij[i] = qij;
+        acc[0] += pij / (U - xv) + qij / (xv - L);
+        m.xo2[i] = m.xo1[i];
+        m.xo1[i] = xv;
+    }
+    rfinish<1>(acc, partials, counter, out);
+}
+void launch_mma_gensub(const MMADev& m, const double* x, const double* df, const double* dg, const double* xmin,
+                       const double* xmax, int k, double asyminit, double asymdec, double asyminc, Reducer& red,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_mma_gensub, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), x, df, dg, xmin, xmax, k, asyminit,
+                       asymdec, asyminc, red.partials.p, red.counter.p, red.scal.p + SC_TMP0);
+    TT_HIP(hipGetLastError());
+}
+
+// XYZofLAMBDA (MMA.cc:684-727) x part, fused with DualGrad (:729-765) and DualHess (:767-840) sums
+__global__ void k_mma_xgh(MD m, double lam, double* __restrict__ x, double* partials, unsigned* counter, double* out) {
+    double acc[2] = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m.n; i += (long)gridDim.x * blockDim.x) {
+        const double pj = m.p0[i] + m.pij[i] * lam, qj = m.q0[i] + m.qij[i] * lam;
+        const double L = m.L[i], U = m.U[i];
+        const double xp = (sqrt(pj) * L + sqrt(qj) * U) / (sqrt(pj) + sqrt(qj));
+        double xv = xp;
+        if (xv < m.alpha[i]) xv = m.alpha[i];
+        if (xv > m.beta[i]) xv = m.beta[i];
+        x[i] = xv;
+        acc[0] += m.pij[i] / (U - xv) + m.qij[i] / (xv - L);
+        const double PQ = m.pij[i] / pow(U - xv, 2.0) - m.qij[i] / pow(xv - L, 2.0);
+        double df2 = -1.0 / (2.0 * pj / pow(U - xv, 3.0) + 2.0 * qj / pow(xv - L, 3.0));
+        if (xp < m.alpha[i]) df2 = 0.0;
+        if (xp > m.beta[i]) df2 = 0.0;
+        acc[1] += (PQ * df2) * PQ;
+    }
+    rfinish<2>(acc, partials, counter, out);
+}
+void launch_mma_xyz_grad_hess(const MMADev& m, double lam, double* x, Reducer& red, hipStream_t s) {
+    hipLaunchKernelGGL(k_mma_xgh, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), lam, x, red.partials.p, red.counter.p,
+                       red.scal.p + SC_TMP0);
+    TT_HIP(hipGetLastError());
+}
+// XYZofLAMBDA + DualResidual sum (MMA.cc:893-934)
+__global__ void k_mma_xres(MD m, double lam, double* __restrict__ x, double* partials, unsigned* counter, double* out) {
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m.n; i += (long)gridDim.x * blockDim.x) {
+        const double pj = m.p0[i] + m.pij[i] * lam, qj = m.q0[i] + m.qij[i] * lam;
+        const double L = m.L[i], U = m.U[i];
+        double xv = (sqrt(pj) * L + sqrt(qj) * U) / (sqrt(pj) + sqrt(qj));
+        if (xv < m.alpha[i]) xv = m.alpha[i];
+        if (xv > m.beta[i]) xv = m.beta[i];
+        x[i] = xv;
+        acc[0] += m.pij[i] / (U - xv) + m.qij[i] / (xv - L);
+    }
+    rfinish<1>(acc, partials, counter, out);
+}
+void launch_mma_xyz_res(const MMADev& m, double lam, double* x, Reducer& red, hipStream_t s) {
+    hipLaunchKernelGGL(k_mma_xres, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), lam, x, red.partials.p, red.counter.p,
+                       red.scal.p + SC_TMP0);
+    TT_HIP(hipGetLastError());
+}
+
+// SetOuterMovelimit (MMA.cc:374-393)
+__global__ void k_mma_movelimit(const double* __restrict__ x, double* __restrict__ xmin, double* __restrict__ xmax,
+                                long n, double Xmin, double Xmax, double movlim) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i] + movlim, b = x[i] - movlim;
+    xmax[i] = Xmax < a ? Xmax : a;
+    xmin[i] = Xmin > b ? Xmin : b;
+}
+void launch_mma_movelimit(const double* x, double* xmin, double* xmax, long n, double Xmin, double Xmax,
+                          double movlim, hipStream_t s) {
+    hipLaunchKernelGGL(k_mma_movelimit, dim3(nblk(n)), dim3(BS2), 0, s, x, xmin, xmax, n, Xmin, Xmax, movlim);
+    TT_HIP(hipGetLastError());
+}
+
+// DesignChange (MMA.cc:395-414): max |x - xold| -> out (pre-zeroed), xold = x
+__global__ void k_mma_change(const double* __restrict__ x, double* __restrict__ xold, long n, double* out) {
+    double m = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        m = fmax(m, fabs(x[i] - xold[i]));
+        xold[i] = x[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out, m);
+}
+void launch_mma_change(const double* x, double* xold, long n, Reducer& red, int slot, hipStream_t s) {
+    TT_HIP(hipMemsetAsync(red.scal.p + slot, 0, sizeof(double), s));
+    hipLaunchKernelGGL(k_mma_change, dim3(rblk(n)), dim3(BS2), 0, s, x, xold, n, red.scal.p + slot);
+    TT_HIP(hipGetLastError());
+}
+
+// KKTresidual element part (MMA.cc:416-470): sum of squares -> scal[TMP1], max abs -> scal[TMP2]
+__global__ void k_mma_kkt(const double* __restrict__ x, const double* __restrict__ df, const double* __restrict__ dg,
+                          const double* __restrict__ xmin, const double* __restrict__ xmax, long n, double lam,
+                          double* partials, unsigned* counter, double* out, double* outmax) {
+    double acc[1] = {0.0};
+    double mx = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double ri = df[i] + lam * dg[i];
+        double mmin = 0.0, mmax = 0.0;
+        if (x[i] < xmin[i] + 1.0e-5 && ri > 0.0) mmin = ri;
+        if (x[i] > xmax[i] - 1.0e-5 && ri < 0.0) mmax = -ri;
+        ri += -mmin + mmax;
+        acc[0] += pow(ri, 2.0);
+        mx = fmax(fabs(ri), mx);
+        double r2 = mmin * (x[i] - xmin[i]);
+        acc[0] += pow(r2, 2.0);
+        mx = fmax(fabs(r2), mx);
+        r2 = mmax * (xmax[i] - x[i]);
+        acc[0] += pow(r2, 2.0);
+        mx = fmax(fabs(r2), mx);
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(outmax, mx);
+    rfinish<1>(acc, partials, counter, out);
+}
+void launch_mma_kkt(const double* x, const double* df, const double* dg, const double* xmin, const double* xmax,
+                    long n, double lam, Reducer& red, hipStream_t s) {
+    TT_HIP(hipMemsetAsync(red.scal.p + SC_TMP2, 0, sizeof(double), s));
+    hipLaunchKernelGGL(k_mma_kkt, dim3(rblk(n)), dim3(BS2), 0, s, x, df, dg, xmin, xmax, n, lam, red.partials.p,
+                       red.counter.p, red.scal.p + SC_TMP1, red.scal.p + SC_TMP2);
+    TT_HIP(hipGetLastError());
+}
+
+__global__ void k_lincomb(double a, const double* x, double b, const double* y, double* out, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + b * y[i];
+}
+void launch_lincomb(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_lincomb, dim3(nblk(n)), dim3(BS2), 0, s, a, x, b, y, out, n);
+    TT_HIP(hipGetLastError());
+}
+
+}  // namespace tt

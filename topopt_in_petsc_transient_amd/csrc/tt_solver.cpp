@@ -1,0 +1,483 @@
+// Multigrid preconditioner and flexible CG.
+//
+// Replaces the reference's KSP set-up (LinearElasticity.cc:2679-2939: FGMRES +
+// PCMG V-cycle with Galerkin coarse operators and GMRES/SOR smoothers) by
+//   * a flexible CG outer iteration in FP64 (Polak-Ribiere form, so that a
+//     variable preconditioner is allowed), stopping on the true
+//     unpreconditioned residual ||b - A x|| <= rtol ||b||;
+//   * a geometric V-cycle on the same DMDA-style hierarchy (nodes coarsened
+//     by 2 per direction, trilinear interpolation, restriction = P^T) with
+//     rediscretised coarse operators (element coefficients averaged over the
+//     8 children), Chebyshev smoothing on D^-1 A, and an exact coarsest solve
+//     with a dense FP64 inverse;
+//   * in the mixed mode the V-cycle runs in FP32 with BF16-stored coarse
+//     coefficients; the outer iteration, its residual and dot products stay
+//     FP64, so the converged answer is an FP64 answer.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tt_context.h"
+
+namespace tt {
+
+static void build_mask(const Grid& g, int boundaries, std::vector<uint8_t>& m) {
+    m.assign(g.nn, 0);
+    for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) {
+                uint8_t b = 0;
+                if (boundaries == 1) {
+                    if (i == 0) b = 7;  // cantilever: x = xmin face fully clamped (LinearElasticity.cc:565-572)
+                } else {
+                    if (i == 0) b |= 3;                   // MBB: x,y at x = xmin (:553-557)
+                    if (i == g.nx - 1 && k == 0) b |= 6;  // y,z on the edge x = xmax, z = zmin (:558-562)
+                }
+                m[i + (long)g.nx * (j + (long)g.ny * k)] = b;
+            }
+}
+
+void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
+    ctx = c;
+    mixed = mixed_prec;
+    cheb_deg = deg;
+    const Grid& g0 = c->g;
+    const int f = 1 << (nlvls - 1);
+    TT_CHECK(nlvls >= 1 && nlvls <= 8, TT_ERR_ARG, "nlvls must be in 1..8");
+    TT_CHECK(g0.ex % f == 0 && g0.ey % f == 0 && g0.ez % f == 0, TT_ERR_ARG,
+             "mesh dimension not compatible with the number of multigrid levels");
+    L.clear();
+    for (int l = 0; l < nlvls; l++) {
+        auto lev = std::make_unique<MGLevel>();
+        const int s = 1 << l;
+        lev->g = make_grid(g0.ex / s, g0.ey / s, g0.ez / s, g0.hx * g0.ex, g0.hy * g0.ey, g0.hz * g0.ez);
+        lev->fp64 = !mixed;
+        const Grid& g = lev->g;
+        std::vector<uint8_t> m;
+        build_mask(g, c->prm.boundaries, m);
+        lev->mask.alloc(g.nn);
+        TT_HIP(hipMemcpy(lev->mask.p, m.data(), g.nn, hipMemcpyHostToDevice));
+        double ke[576], m8[64];
+        hex8_stiffness(g.hx, g.hy, g.hz, c->prm.nu, ke);
+        hex8_mass(g.hx, g.hy, g.hz, m8);
+        float ke32[576], m832[64];
+        for (int i = 0; i < 576; i++) ke32[i] = (float)ke[i];
+        for (int i = 0; i < 64; i++) m832[i] = (float)m8[i];
+        lev->KE64.alloc(576);
+        lev->M864.alloc(64);
+        lev->KE32.alloc(576);
+        lev->M832.alloc(64);
+        TT_HIP(hipMemcpy(lev->KE64.p, ke, sizeof(ke), hipMemcpyHostToDevice));
+        TT_HIP(hipMemcpy(lev->M864.p, m8, sizeof(m8), hipMemcpyHostToDevice));
+        TT_HIP(hipMemcpy(lev->KE32.p, ke32, sizeof(ke32), hipMemcpyHostToDevice));
+        TT_HIP(hipMemcpy(lev->M832.p, m832, sizeof(m832), hipMemcpyHostToDevice));
+        if (l > 0) {
+            lev->kf64.alloc(g.nel);
+            lev->mf64.alloc(g.nel);
+            if (mixed) {
+                lev->kfbf.alloc(g.nel);
+                lev->mfbf.alloc(g.nel);
+            }
+        } else if (mixed) {
+            lev->kf32.alloc(g.nel);
+            lev->mf32.alloc(g.nel);
+        }
+        const size_t es = lev->esz();
+        for (DBuf<char>* b : {&lev->b, &lev->x, &lev->x2, &lev->r, &lev->d, &lev->d2, &lev->dinv})
+            b->alloc(g.nd * es);
+        lev->op.g = g;
+        lev->op.mask = lev->mask.p;
+        lev->op.KE64 = lev->KE64.p;
+        lev->op.M864 = lev->M864.p;
+        lev->op.KE32 = lev->KE32.p;
+        lev->op.M832 = lev->M832.p;
+        L.push_back(std::move(lev));
+    }
+    const Grid& gc = L.back()->g;
+    nc = (int)gc.nd;
+    dense = nc <= 2400;
+    cb.alloc(nc);
+    cx.alloc(nc);
+    if (dense) {
+        Ainv.alloc((size_t)nc * nc);
+    } else {
+        cdinv.alloc(nc);
+        cx2.alloc(nc);
+        cd.alloc(nc);
+        cd2.alloc(nc);
+    }
+}
+
+void Multigrid::set_coefficients(const double* kf_fine, const double* mf_fine) {
+    hipStream_t s = ctx->stream;
+    MGLevel& l0 = *L[0];
+    if (mixed) {
+        launch_convert_coef(kf_fine, l0.kf32.p, l0.g.nel, COEF_F32, s);
+        launch_convert_coef(mf_fine, l0.mf32.p, l0.g.nel, COEF_F32, s);
+        l0.op.coef_kind = COEF_F32;
+        l0.op.kf = l0.kf32.p;
+        l0.op.mf = l0.mf32.p;
+    } else {
+        l0.op.coef_kind = COEF_F64;
+        l0.op.kf = kf_fine;
+        l0.op.mf = mf_fine;
+    }
+    const double* pk = kf_fine;
+    const double* pm = mf_fine;
+    for (size_t l = 1; l < L.size(); l++) {
+        MGLevel& lv = *L[l];
+        launch_coarsen_coef(L[l - 1]->g, lv.g, pk, pm, lv.kf64.p, lv.mf64.p, s);
+        pk = lv.kf64.p;
+        pm = lv.mf64.p;
+        if (mixed) {
+            launch_convert_coef(lv.kf64.p, lv.kfbf.p, lv.g.nel, COEF_BF16, s);
+            launch_convert_coef(lv.mf64.p, lv.mfbf.p, lv.g.nel, COEF_BF16, s);
+            lv.op.coef_kind = COEF_BF16;
+            lv.op.kf = lv.kfbf.p;
+            lv.op.mf = lv.mfbf.p;
+        } else {
+            lv.op.coef_kind = COEF_F64;
+            lv.op.kf = lv.kf64.p;
+            lv.op.mf = lv.mf64.p;
+        }
+    }
+}
+
+// largest eigenvalue of D^-1 A by power iteration (deterministic start vector)
+double Multigrid::power_lmax(int l, int iters) {
+    MGLevel& lv = *L[l];
+    hipStream_t s = ctx->stream;
+    const long nd = lv.g.nd;
+    std::vector<double> h(nd);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (long i = 0; i < nd; i++) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        h[i] = ((double)(st >> 11) / 9007199254740992.0) * 2.0 - 1.0;
+    }
+    if (lv.fp64) {
+        TT_HIP(hipMemcpyAsync(lv.x.p, h.data(), nd * 8, hipMemcpyHostToDevice, s));
+    } else {
+        std::vector<float> hf(h.begin(), h.end());
+        TT_HIP(hipMemcpyAsync(lv.x.p, hf.data(), nd * 4, hipMemcpyHostToDevice, s));
+        TT_HIP(hipStreamSynchronize(s));
+    }
+    TT_HIP(hipStreamSynchronize(s));
+    for (int it = 0; it < iters; it++) {
+        launch_apply_bc(lv.op, cK, cM, lv.x.p, lv.r.p, lv.fp64, s);
+        launch_norm_power(lv.r.p, lv.dinv.p, lv.x.p, nd, ctx->red, lv.fp64, s);
+        launch_scale_by_scal(lv.x.p, nd, ctx->red.scal.p, SC_TMP0, lv.fp64, s);
+    }
+    return std::sqrt(ctx->read_scal(SC_TMP0));
+}
+
+// Cholesky-based inverse of the dense SPD coarsest operator (host, FP64, row-major, contiguous inner loops)
+static void spd_inverse(std::vector<double>& A, int n) {
+    for (int j = 0; j < n; j++) {  // A = L L^T in the lower triangle
+        double* Aj = &A[(size_t)j * n];
+        double d = Aj[j];
+        for (int k = 0; k < j; k++) d -= Aj[k] * Aj[k];
+        TT_CHECK(d > 0.0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
+        d = std::sqrt(d);
+        Aj[j] = d;
+        for (int i = j + 1; i < n; i++) {
+            double* Ai = &A[(size_t)i * n];
+            double s = Ai[j];
+            for (int k = 0; k < j; k++) s -= Ai[k] * Aj[k];
+            Ai[j] = s / d;
+        }
+    }
+    std::vector<double> W((size_t)n * n, 0.0);  // W = L^-1, row by row
+    for (int i = 0; i < n; i++) {
+        const double* Ai = &A[(size_t)i * n];
+        double* Wi = &W[(size_t)i * n];
+        Wi[i] = 1.0;
+        for (int m = 0; m < i; m++) {
+            const double l = Ai[m];
+            const double* Wm = &W[(size_t)m * n];
+            for (int k = 0; k <= m; k++) Wi[k] -= l * Wm[k];
+        }
+        const double inv = 1.0 / Ai[i];
+        for (int k = 0; k <= i; k++) Wi[k] *= inv;
+    }
+    std::fill(A.begin(), A.end(), 0.0);  // A^-1 = W^T W, accumulated by rows of W
+    for (int k = 0; k < n; k++) {
+        const double* Wk = &W[(size_t)k * n];
+        for (int i = 0; i <= k; i++) {
+            const double w = Wk[i];
+            double* Ai = &A[(size_t)i * n];
+            for (int j = 0; j <= i; j++) Ai[j] += w * Wk[j];
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++) A[(size_t)j * n + i] = A[(size_t)i * n + j];
+}
+
+void Multigrid::assemble_coarse_inverse() {
+    MGLevel& lc = *L.back();
+    const Grid& g = lc.g;
+    std::vector<double> kf(g.nel), mf(g.nel);
+    const double* kd = L.size() > 1 ? lc.kf64.p : (const double*)lc.op.kf;
+    const double* md = L.size() > 1 ? lc.mf64.p : (const double*)lc.op.mf;
+    if (L.size() == 1 && mixed) {  // single-level mixed run: use the context's FP64 coefficients
+        kd = ctx->kf.p;
+        md = ctx->mf.p;
+    }
+    TT_HIP(hipMemcpyAsync(kf.data(), kd, g.nel * 8, hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipMemcpyAsync(mf.data(), md, g.nel * 8, hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<uint8_t> m;
+    build_mask(g, ctx->prm.boundaries, m);
+    double ke[576], m8[64];
+    hex8_stiffness(g.hx, g.hy, g.hz, ctx->prm.nu, ke);
+    hex8_mass(g.hx, g.hy, g.hz, m8);
+    const int n = nc;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (long e = 0; e < g.nel; e++) {
+        const long i = e % g.ex, j = (e / g.ex) % g.ey, k = e / ((long)g.ex * g.ey);
+        const long n0 = i + (long)g.nx * (j + (long)g.ny * k), sy = g.nx, sz = (long)g.nx * g.ny;
+        const long nodes[8] = {n0, n0 + 1, n0 + 1 + sy, n0 + sy, n0 + sz, n0 + 1 + sz, n0 + 1 + sy + sz, n0 + sy + sz};
+        const double ck = cK * kf[e], cm = cM * mf[e];
+        for (int p = 0; p < 24; p++)
+            for (int q = 0; q < 24; q++) {
+                const int a = p / 3, b = q / 3;
+                double v = ck * ke[24 * p + q] + ((p % 3) == (q % 3) ? cm * m8[8 * a + b] : 0.0);
+                A[(size_t)(3 * nodes[a] + p % 3) * n + (3 * nodes[b] + q % 3)] += v;
+            }
+    }
+    for (long nn = 0; nn < g.nn; nn++)
+        for (int c = 0; c < 3; c++)
+            if ((m[nn] >> c) & 1) {
+                const long d = 3 * nn + c;
+                for (int q = 0; q < n; q++) {
+                    A[(size_t)d * n + q] = 0.0;
+                    A[(size_t)q * n + d] = 0.0;
+                }
+                A[(size_t)d * n + d] = 1.0;
+            }
+    spd_inverse(A, n);
+    TT_HIP(hipMemcpyAsync(Ainv.p, A.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void Multigrid::setup_operator(double cK_, double cM_) {
+    cK = cK_;
+    cM = cM_;
+    hipStream_t s = ctx->stream;
+    const int nl = (int)L.size();
+    for (int l = 0; l < nl - 1; l++) {
+        MGLevel& lv = *L[l];
+        launch_diag_inv(lv.op, cK, cM, lv.dinv.p, lv.fp64, s);
+        lv.lmax = power_lmax(l, 12);
+        if (l < 8) ctx->stats.lambda_max[l] = lv.lmax;
+    }
+    ctx->stats.nlvls = nl;
+    if (dense) {
+        assemble_coarse_inverse();
+    } else {
+        // Chebyshev coarse solve in FP64 on [clmax/30, 1.05 clmax] of D^-1 A
+        OpLevel op = coarse_op64();
+        launch_diag_inv(op, cK, cM, cdinv.p, true, s);
+        std::vector<double> h(nc);
+        uint64_t st = 0x2545F4914F6CDD1Dull;
+        for (int i = 0; i < nc; i++) {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            h[i] = ((double)(st >> 11) / 9007199254740992.0) * 2.0 - 1.0;
+        }
+        TT_HIP(hipMemcpyAsync(cx.p, h.data(), nc * 8, hipMemcpyHostToDevice, s));
+        TT_HIP(hipStreamSynchronize(s));
+        for (int it = 0; it < 20; it++) {
+            launch_apply_bc(op, cK, cM, cx.p, cd.p, true, s);
+            launch_norm_power(cd.p, cdinv.p, cx.p, nc, ctx->red, true, s);
+            launch_scale_by_scal(cx.p, nc, ctx->red.scal.p, SC_TMP0, true, s);
+        }
+        clmax = 1.05 * std::sqrt(ctx->read_scal(SC_TMP0));
+        clmin = clmax / 30.0;
+    }
+}
+
+OpLevel Multigrid::coarse_op64() const {
+    const MGLevel& lc = *L.back();
+    OpLevel op = lc.op;
+    op.coef_kind = COEF_F64;
+    if (L.size() > 1) {
+        op.kf = lc.kf64.p;
+        op.mf = lc.mf64.p;
+    } else {
+        op.kf = ctx->kf.p;
+        op.mf = ctx->mf.p;
+    }
+    return op;
+}
+
+void Multigrid::coarse_solve() {
+    MGLevel& lc = *L.back();
+    hipStream_t s = ctx->stream;
+    const double* b64 = (const double*)lc.b.p;
+    double* x64 = (double*)lc.x.p;
+    if (!lc.fp64) {
+        launch_copy(lc.b.p, cb.p, nc, false, true, s);
+        b64 = cb.p;
+        x64 = cx.p;
+    }
+    if (dense) {
+        launch_dense_gemv(Ainv.p, b64, x64, nc, s);
+    } else {
+        OpLevel op = coarse_op64();
+        const double theta = 0.5 * (clmax + clmin), delta = 0.5 * (clmax - clmin), sigma = theta / delta;
+        double* x = cx.p;
+        double* x2 = cx2.p;
+        double* d = cd.p;
+        double* d2 = cd2.p;
+        launch_cheb_first(b64, cdinv.p, 1.0 / theta, x, nc, true, s);
+        double rho0 = 1.0 / sigma;
+        for (int k = 1; k < coarse_deg; k++) {
+            const double rho1 = 1.0 / (2.0 * sigma - rho0);
+            launch_cheb_step(op, cK, cM, b64, x, k == 1 ? x : d, cdinv.p, rho1 * rho0, 2.0 * rho1 / delta, x2, d2,
+                             true, s);
+            std::swap(x, x2);
+            std::swap(d, d2);
+            rho0 = rho1;
+        }
+        if (lc.fp64) launch_copy(x, x64, nc, true, true, s);
+        else x64 = x;
+    }
+    if (!lc.fp64) launch_copy(x64, lc.x.p, nc, true, false, s);
+}
+
+void Multigrid::vcycle(int l) {
+    const int nl = (int)L.size();
+    if (l == nl - 1) {
+        coarse_solve();
+        return;
+    }
+    MGLevel& lv = *L[l];
+    hipStream_t s = ctx->stream;
+    const bool f64 = lv.fp64;
+    const long nd = lv.g.nd;
+    const double lo = cheb_lo * lv.lmax, hi = cheb_hi * lv.lmax;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    // pre-smoothing from x = 0
+    launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
+    double rho0 = 1.0 / sigma;
+    bool d_is_x = true;
+    for (int k = 1; k < cheb_deg; k++) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        launch_cheb_step(lv.op, cK, cM, lv.b.p, lv.x.p, d_is_x ? lv.x.p : lv.d.p, lv.dinv.p, rho1 * rho0,
+                         2.0 * rho1 / delta, lv.x2.p, lv.d2.p, f64, s);
+        std::swap(lv.x.p, lv.x2.p);
+        std::swap(lv.d.p, lv.d2.p);
+        d_is_x = false;
+        rho0 = rho1;
+    }
+    // coarse-grid correction
+    launch_residual(lv.op, cK, cM, lv.b.p, lv.x.p, lv.r.p, f64, s);
+    MGLevel& lc = *L[l + 1];
+    launch_restrict(lv.g, lc.g, lv.r.p, lc.mask.p, lc.b.p, f64, s);
+    vcycle(l + 1);
+    launch_prolong_add(lv.g, lc.g, lc.x.p, lv.mask.p, lv.x.p, f64, s);
+    // post-smoothing
+    rho0 = 1.0 / sigma;
+    for (int k = 0; k < cheb_deg; k++) {
+        double c0 = 0.0, c1 = 1.0 / theta;
+        if (k > 0) {
+            const double rho1 = 1.0 / (2.0 * sigma - rho0);
+            c0 = rho1 * rho0;
+            c1 = 2.0 * rho1 / delta;
+            rho0 = rho1;
+        }
+        launch_cheb_step(lv.op, cK, cM, lv.b.p, lv.x.p, k > 0 ? lv.d.p : nullptr, lv.dinv.p, c0, c1, lv.x2.p,
+                         lv.d2.p, f64, s);
+        std::swap(lv.x.p, lv.x2.p);
+        std::swap(lv.d.p, lv.d2.p);
+    }
+}
+
+void Multigrid::apply(const double* r, double* z) {
+    MGLevel& l0 = *L[0];
+    hipStream_t s = ctx->stream;
+    const long nd = l0.g.nd;
+    launch_copy(r, l0.b.p, nd, true, l0.fp64, s);
+    vcycle(0);
+    launch_copy(l0.x.p, z, nd, l0.fp64, true, s);
+}
+
+// ------------------------------------------------------------------ context-level Krylov
+void Context::sync() { TT_HIP(hipStreamSynchronize(stream)); }
+
+double Context::read_scal(int slot) {
+    TT_HIP(hipMemcpyAsync(red.host + slot, red.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, stream));
+    TT_HIP(hipStreamSynchronize(stream));
+    return red.host[slot];
+}
+
+OpLevel Context::fine_op(bool fp64) const {
+    OpLevel o = op_fine64;
+    if (!fp64) {
+        o.coef_kind = COEF_F32;
+        o.kf = kf32.p;
+        o.mf = mf32.p;
+    }
+    return o;
+}
+
+SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, double* x) {
+    const long nd = g.nd;
+    const OpLevel& op = op_fine64;
+    const double rtol = prm.rtol;
+    launch_residual(op, cK, cM, b, x, kr.p, true, stream);
+    launch_dot(b, b, nd, red, SC_TMP1, stream);
+    launch_dot(kr.p, kr.p, nd, red, SC_RR, stream);
+    TT_HIP(hipMemcpyAsync(red.host, red.scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, stream));
+    sync();
+    const double bn = std::sqrt(red.host[SC_TMP1]);
+    SolveResult res;
+    stats.solves++;
+    if (!std::isfinite(bn)) throw Error{TT_ERR_NONFINITE, "non-finite right-hand side"};
+    if (bn == 0.0) {
+        launch_fill(x, 0.0, nd, true, stream);
+        stats.last_iters = 0;
+        stats.last_relres = 0.0;
+        return res;
+    }
+    double rel = std::sqrt(red.host[SC_RR]) / bn;
+    bool first = true, conv = rel <= rtol;
+    double* pold = kp0.p;
+    double* pnew = kp1.p;
+    int it = 0;
+    while (!conv && it < prm.maxit) {
+        if (use_mg) mg.apply(kr.p, kz.p);
+        else launch_jacobi(dinvM.p, kr.p, kz.p, nd, stream);
+        launch_fcg_dots(kz.p, kr.p, krp.p, nd, red, first, stream);
+        launch_fcg_apply(op, cK, cM, kz.p, pold, pnew, kq.p, red, first, stream);
+        launch_fcg_update(x, pnew, kr.p, krp.p, kq.p, nd, red, stream);
+        std::swap(pold, pnew);
+        first = false;
+        it++;
+        rel = std::sqrt(read_scal(SC_RR)) / bn;
+        if (!std::isfinite(rel)) throw Error{TT_ERR_NONFINITE, "non-finite residual in flexible CG"};
+        if (rel <= rtol) {
+            // confirm with the true residual; restart the recursion if it drifted
+            launch_residual(op, cK, cM, b, x, kr.p, true, stream);
+            launch_dot(kr.p, kr.p, nd, red, SC_RR, stream);
+            rel = std::sqrt(read_scal(SC_RR)) / bn;
+            if (rel <= rtol) conv = true;
+            else first = true;
+        }
+    }
+    stats.krylov_iters += it;
+    stats.last_iters = it;
+    stats.last_relres = rel;
+    stats.max_relres = std::max(stats.max_relres, rel);
+    res.iters = it;
+    res.relres = rel;
+    if (!conv)
+        throw Error{TT_ERR_NOT_CONVERGED, "flexible CG did not reach rtol " + std::to_string(rtol) + " in " +
+                                              std::to_string(prm.maxit) + " iterations (relres " +
+                                              std::to_string(rel) + ")"};
+    return res;
+}
+
+SolveResult Context::solve_A(const double* b, double* x) { return fcg(cK_A, cM_A, true, b, x); }
+SolveResult Context::solve_M(const double* b, double* x) { return fcg(0.0, 1.0, false, b, x); }
+
+}  // namespace tt

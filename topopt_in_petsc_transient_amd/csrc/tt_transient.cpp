@@ -1,0 +1,323 @@
+// Newmark forward sweep, binomial checkpointing and the backward adjoint sweep.
+//
+// Forward: CompleteInitialState (LinearElasticity.cc:956-1206) and
+// SolveStateBlock (:1298-1397).  Per step the right-hand side
+//   f(t) + B1 u + B2 v + B3 a,  B1 = a5 M + a2 C, B2 = a3 M - a0 C, B3 = a4 M - a1 C
+// is one fused matrix-free pass (K and M applied to two combinations of u, v, a),
+// followed by one MG-preconditioned flexible-CG solve with (K + a2 C + a5 M)
+// and the Newmark corrector.
+//
+// Adjoint: SolveLagrangen (:808-954) and SolveLagrange0 (:678-804), with the
+// sensitivities of ComputeSensitivitiesElementLoopn/0 (:1911-2204) and the
+// objective of ComputeObjectiveElementLoop (:2359-2452) evaluated in the same
+// backward pass.  The reference keeps ncp*tsc states on disk/in RAM; here the
+// states live in HBM, either all of them or a binomial set of checkpoints with
+// recomputation between them (Griewank's revolve schedule).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "tt_context.h"
+
+namespace tt {
+
+double Context::load_intensity(int step) const {
+    double s;
+    if (prm.loadsignal == 0) {
+        // LinearElasticity.cc:598-609 at t = step*dt
+        const double t = step * dt, t0 = 8.0, c = 1.1;
+        s = 1.0 * std::exp(-std::pow(t - t0, 2) / std::pow(2 * c, 2)) * std::cos(4.0 * (t - t0)) *
+            std::cos(5.0 * (t - t0)) * std::cos(6.0 * (t - t0)) * std::cos(7.0 * (t - t0));
+    } else {
+        s = (step <= prm.pulse_steps) ? std::sin(M_PI * (double)step / (double)prm.pulse_steps) : 0.0;
+    }
+    return s / (double)prm.nely;  // LoadIntensity = signal/(nn[1]-1) (:611)
+}
+
+// ------------------------------------------------------------------ forward
+void Context::complete_initial_state() {
+    TT_CHECK(density_set, TT_ERR_STATE, "tt_set_density must be called first");
+    const long nd = g.nd;
+    mg.setup_operator(cK_A, cM_A);
+    launch_diag_inv(op_fine64, 0.0, 1.0, dinvM.p, true, stream);
+    // state 0: u0 = v0 = 0 (LinearElasticity.cc:521-525); M~ a0 = N.(f(0) - C v0 - K u0) (:1050-1078)
+    double* s0 = slots[0].p;
+    launch_fill(s0, 0.0, 2 * nd, true, stream);
+    const double wK[3] = {-1.0, -prm.dbeta, 0.0};
+    const double wM[3] = {0.0, -prm.dalpha, 0.0};
+    launch_newmark_rhs(op_fine64, s0, s0 + nd, s0 + 2 * nd, F.p, load_intensity(0), wK, wM, rhs.p, stream);
+    launch_fill(s0 + 2 * nd, 0.0, nd, true, stream);
+    solve_M(rhs.p, s0 + 2 * nd);
+    for (auto& v : slot_step) v = -1;
+    slot_step[0] = 0;
+    initial_done = true;
+    forward_done = false;
+}
+
+void Context::newmark_step(const double* s_in, double* s_out, int t) {
+    const long nd = g.nd;
+    const double* u = s_in;
+    const double* v = s_in + nd;
+    const double* ac = s_in + 2 * nd;
+    double* un = s_out;
+    double* vn = s_out + nd;
+    double* an = s_out + 2 * nd;
+    const double al = prm.dalpha, be = prm.dbeta;
+    const double wK[3] = {a[2] * be, -a[0] * be, -a[1] * be};
+    const double wM[3] = {a[5] + a[2] * al, a[3] - a[0] * al, a[4] - a[1] * al};
+    launch_newmark_rhs(op_fine64, u, v, ac, F.p, load_intensity(t), wK, wM, rhs.p, stream);
+    launch_copy(u, un, nd, true, true, stream);  // initial guess u_n (:1358)
+    solve_A(rhs.p, un);
+    launch_newmark_update(u, v, ac, un, vn, an, a, nd, stream);
+    stats.forward_steps++;
+}
+
+static long binom(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    long r = 1;
+    for (int i = 1; i <= k; i++) {
+        r = r * (n - k + i) / i;
+        if (r > (1L << 40)) return 1L << 40;
+    }
+    return r;
+}
+
+// right-part length of a binomial split of n steps with s free slots
+static int revolve_right(int n, int s) {
+    int t = 1;
+    while (binom(s + t, s) < n) t++;
+    long right = binom(s - 1 + t, s - 1);  // beta(s-1, t)
+    if (right > n - 1) right = n - 1;
+    if (right < 1) right = 1;
+    return (int)right;
+}
+
+struct Revolve {
+    Context* c;
+    std::function<void(int)> adjoint_at;  // working state st_cur holds state t-1
+    int find_slot(int step) const {
+        for (size_t i = 0; i < c->slot_step.size(); i++)
+            if (c->slot_step[i] == step) return (int)i;
+        return -1;
+    }
+    int free_slot() const {
+        for (size_t i = 1; i < c->slot_step.size(); i++)
+            if (c->slot_step[i] < 0) return (int)i;
+        return -1;
+    }
+    void restore(int step) {
+        const int k = find_slot(step);
+        TT_CHECK(k >= 0, TT_ERR_INTERNAL, "revolve: checkpoint missing");
+        launch_copy(c->slots[k].p, c->st_cur.p, 3 * c->g.nd, true, true, c->stream);
+    }
+    void advance(int from, int to) {  // st_cur: state from -> state to
+        for (int t = from + 1; t <= to; t++) {
+            c->newmark_step(c->st_cur.p, c->st_next.p, t);
+            std::swap(c->st_cur.p, c->st_next.p);
+        }
+    }
+    void shot(int step) {
+        int k = find_slot(step);
+        if (k >= 0) return;
+        k = free_slot();
+        TT_CHECK(k >= 0, TT_ERR_INTERNAL, "revolve: no free checkpoint slot");
+        launch_copy(c->st_cur.p, c->slots[k].p, 3 * c->g.nd, true, true, c->stream);
+        c->slot_step[k] = step;
+    }
+    void release(int step) {
+        const int k = find_slot(step);
+        if (k > 0) c->slot_step[k] = -1;
+    }
+    // reverse steps r, r-1, ..., l+1; state l is checkpointed; s free slots
+    void reverse(int l, int r, int s) {
+        if (r - l == 1) {
+            restore(l);
+            adjoint_at(r);
+            return;
+        }
+        if (s == 0) {
+            for (int t = r; t > l; t--) {
+                restore(l);
+                advance(l, t - 1);
+                adjoint_at(t);
+            }
+            return;
+        }
+        const int m = r - revolve_right(r - l, s);
+        if (find_slot(m) < 0) {
+            // nearest stored state at or below m
+            int best = l;
+            for (size_t i = 0; i < c->slot_step.size(); i++)
+                if (c->slot_step[i] > best && c->slot_step[i] <= m) best = c->slot_step[i];
+            restore(best);
+            advance(best, m);
+            shot(m);
+        }
+        reverse(m, r, s - 1);
+        release(m);
+        reverse(l, m, s);
+    }
+    // states stored by the forward sweep: the split points of the first descent
+    static void first_descent(int n, int s, std::vector<int>& pts) {
+        int l = 0, r = n;
+        while (r - l > 1 && s > 0) {
+            const int m = r - revolve_right(r - l, s);
+            pts.push_back(m);
+            l = m;
+            s--;
+        }
+    }
+};
+
+void Context::forward() {
+    TT_CHECK(initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
+    const int T = prm.nsteps;
+    const long nd3 = 3 * g.nd;
+    for (size_t i = 1; i < slot_step.size(); i++) slot_step[i] = -1;
+    launch_copy(slots[0].p, st_cur.p, nd3, true, true, stream);
+    if (store_all) {
+        for (int t = 1; t <= T; t++) {
+            newmark_step(slots[t - 1].p, slots[t].p, t);
+            slot_step[t] = t;
+        }
+        launch_copy(slots[T].p, st_cur.p, nd3, true, true, stream);
+    } else {
+        std::vector<int> pts;
+        Revolve::first_descent(T, (int)slots.size() - 1, pts);
+        int cur = 0;
+        size_t k = 0;
+        for (int t = 1; t <= T; t++) {
+            newmark_step(st_cur.p, st_next.p, t);
+            std::swap(st_cur.p, st_next.p);
+            cur = t;
+            if (k < pts.size() && pts[k] == cur) {
+                launch_copy(st_cur.p, slots[k + 1].p, nd3, true, true, stream);
+                slot_step[k + 1] = cur;
+                k++;
+            }
+        }
+    }
+    sync();
+    forward_done = true;
+}
+
+// ------------------------------------------------------------------ adjoint
+void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
+    TT_CHECK(forward_done, TT_ERR_STATE, "tt_forward must be called first");
+    const int T = prm.nsteps;
+    const long nd = g.nd, nn = g.nn;
+    const double al = prm.dalpha, be = prm.dbeta;
+    const int obj = prm.objective;
+    const double coef[8] = {a[0], a[1], a[2], a[3], a[4], a[5], al, be};
+    launch_fill(df.p, 0.0, g.nel, true, stream);
+    launch_fill(hdu.p, 0.0, nd, true, stream);
+    launch_fill(hddu.p, 0.0, nd, true, stream);
+    launch_fill(hdddu.p, 0.0, nd, true, stream);
+    launch_fill(mun.p, 0.0, nd, true, stream);
+    launch_fill(nun.p, 0.0, nd, true, stream);
+
+    // d(objective)/d(state) (ComputeGradientObjectiveElementLoop + GradObjectiveWrtState, :2206-2356, :1516-1576)
+    auto grad_state = [&](const double* s) {
+        const double* u = s;
+        const double* v = s + nd;
+        const double* ac = s + 2 * nd;
+        switch (obj) {
+        case 0: launch_hdiag(u, rcnt.p, dt, hdu.p, nn, stream); break;
+        case 1: launch_hdiag(v, rcnt.p, dt, hddu.p, nn, stream); break;
+        case 2: launch_hdiag(ac, rcnt.p, dt, hdddu.p, nn, stream); break;
+        case 3: launch_apply_raw(op_fine64, Pkf.p, Pmf.p, dt, 0.0, u, hdu.p, stream); break;
+        case 4: launch_apply_raw(op_fine64, Pkf.p, Pmf.p, 0.0, dt, v, hddu.p, stream); break;
+        case 5:
+            launch_apply_raw(op_fine64, Pkf.p, Pmf.p, dt, 0.0, u, hdu.p, stream);
+            launch_apply_raw(op_fine64, Pkf.p, Pmf.p, 0.0, dt, v, hddu.p, stream);
+            break;
+        default: launch_apply_raw(op_fine64, Pkf.p, Pmf.p, dt * be, dt * al, v, hddu.p, stream); break;
+        }
+    };
+
+    auto sens = [&](int mode, const double* lamv, const double* st, const double* stb, int t) {
+        SensArgs sa;
+        sa.g = g;
+        sa.KE = KE64d.p;
+        sa.M8 = M864d.p;
+        sa.kf = kf.p; sa.mf = mf.p; sa.dkf = dkf.p; sa.dmf = dmf.p; sa.P = P.p;
+        sa.lam = lamv;
+        sa.u = st; sa.v = st + nd; sa.a = st + 2 * nd;
+        sa.ub = stb ? stb : st;
+        sa.vb = stb ? stb + nd : st + nd;
+        sa.ab = stb ? stb + 2 * nd : st + 2 * nd;
+        if (mode == 0) {
+            const double wk[4] = {-(1.0 + a[2] * be), a[2] * be, -a[0] * be, -a[1] * be};
+            const double wm[4] = {-(a[5] + a[2] * al), a[5] + a[2] * al, a[3] - a[0] * al, a[4] - a[1] * al};
+            std::memcpy(sa.wK, wk, sizeof(wk));
+            std::memcpy(sa.wM, wm, sizeof(wm));
+        } else {
+            const double wk[4] = {1.0, be, 0.0, 0.0};
+            const double wm[4] = {0.0, al, 1.0, 0.0};
+            std::memcpy(sa.wK, wk, sizeof(wk));
+            std::memcpy(sa.wM, wm, sizeof(wm));
+        }
+        sa.dt = dt; sa.dalpha = al; sa.dbeta = be;
+        sa.objective = obj;
+        sa.mode = mode;
+        sa.df = df.p;
+        launch_sens(sa, red, fxs.p + t, stream);
+    };
+
+    // one backward step at t: needs state t and state t-1
+    auto adjoint_step = [&](int t, const double* st, const double* stb) {
+        grad_state(st);
+        if (t == T) {
+            launch_adjoint_combine(nullptr, nullptr, hdu.p, hddu.p, hdddu.p, mun.p, nun.p, mask.p, coef, true, mu.p,
+                                   nu.p, rhs.p, nn, stream);
+            // the terminal multiplier of the previous call is the initial guess (:877-883)
+        } else {
+            launch_apply_split(op_fine64, 1.0, 1.0, lamn.p, Kl.p, Ml.p, stream);
+            launch_adjoint_combine(Kl.p, Ml.p, hdu.p, hddu.p, hdddu.p, mun.p, nun.p, mask.p, coef, false, mu.p, nu.p,
+                                   rhs.p, nn, stream);
+            launch_copy(lamn.p, lam.p, nd, true, true, stream);  // initial guess lambda_{t+1} (:917)
+        }
+        solve_A(rhs.p, lam.p);
+        sens(0, lam.p, st, stb, t);
+        std::swap(lam.p, lamn.p);
+        std::swap(mu.p, mun.p);
+        std::swap(nu.p, nun.p);
+        stats.adjoint_steps++;
+        // keep the terminal multiplier as next call's starting guess
+        if (t == T) launch_copy(lamn.p, lamT.p, nd, true, true, stream);
+    };
+
+    launch_copy(lamT.p, lam.p, nd, true, true, stream);
+    if (store_all) {
+        for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
+    } else {
+        Revolve rv;
+        rv.c = this;
+        rv.adjoint_at = [&](int t) {
+            // st_cur = state t-1; take one forward step into st_next = state t
+            newmark_step(st_cur.p, st_next.p, t);
+            adjoint_step(t, st_next.p, st_cur.p);
+        };
+        rv.reverse(0, T, (int)slots.size() - 1);
+    }
+    // t = 0 (SolveLagrange0, :678-804; ComputeSensitivitiesElementLoop0, :2080-2204)
+    const double* s0 = slots[0].p;
+    grad_state(s0);
+    launch_apply_split(op_fine64, 1.0, 1.0, lamn.p, Kl.p, Ml.p, stream);
+    launch_adjoint0_rhs(Kl.p, Ml.p, hdddu.p, mun.p, nun.p, mask.p, coef, rhs.p, nn, stream);
+    solve_M(rhs.p, nu0.p);
+    sens(1, nu0.p, s0, nullptr, 0);
+    // objective: sum over t = 1..T, then t = 0 (:1778-1802)
+    std::vector<double> f(T + 1);
+    TT_HIP(hipMemcpyAsync(f.data(), fxs.p, sizeof(double) * (T + 1), hipMemcpyDeviceToHost, stream));
+    if (dfdx_dev) launch_copy(df.p, dfdx_dev, g.nel, true, true, stream);
+    sync();
+    double fx = 0.0;
+    for (int t = 1; t <= T; t++) fx += f[t];
+    fx += f[0];
+    if (!std::isfinite(fx)) throw Error{TT_ERR_NONFINITE, "non-finite objective"};
+    *fx_host = fx;
+}
+
+}  // namespace tt
