@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the transient-elasticity hot path on one MI355X.
+
+Metric: Newmark steps/s + adjoint-gradient time, 128x64x64 hex8, mixed vs FP64 GPU.
+
+A bench "step" is one Newmark time step of the mixed-precision forward sweep
+(FP32 multigrid with BF16 coarse coefficients inside FP64 flexible CG, rtol
+1e-8) on the 128x64x64 cantilever with dt = 0.005 and a filtered-noise density
+of mean 0.12.  K steps are timed after W warmup steps.  The same run also
+reports:
+  * the all-FP64 forward on the same steps (speed-up of mixed over FP64);
+  * the adjoint gradient (forward 200 steps + backward sweep with binomial
+    checkpointing in 20 state slots) in both modes, with the relative errors of
+    the mixed objective and gradient against the FP64 run;
+  * the operator microbenchmark (K u, M u, A u in FP64 and FP32, 1000 applies
+    each) with its roofline fraction, timed with HIP events on the library's
+    stream;
+  * the CPU oracle on a bounded sample of the same forward sweep.
+Data is synthetic (seeded); nothing is read from disk.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Newmark steps/s + adjoint-gradient time, 128x64x64 hex8, mixed vs FP64 GPU"
+PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
+PEAK_FP64 = 78.6e12        # MI355X dense FP64 (vector and matrix), flop/s
+PEAK_FP32 = 157.3e12       # MI355X dense FP32 matrix, flop/s
+NEL = (128, 64, 64)
+DT = 0.005
+
+
+def params(capi, precision, nsteps, slots=0, nlvls=None):
+    return capi.default_params(
+        nelx=NEL[0], nely=NEL[1], nelz=NEL[2], xc=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0), nsteps=nsteps, t1=DT * nsteps,
+        pulse_steps=20, precision=precision, rtol=1e-8, maxit=200, nlvls=nlvls or int(os.environ.get("TT_NLVLS", 5)),
+        cheb_degree=2, checkpoint_slots=slots, rmin=0.08, filter_rtol=1e-10)
+
+
+def density(capi):
+    """Seeded U[0,1] noise -> PDE filter (rmin 0.08) -> mean 0.12 -> clamp [0.001, 1]."""
+    ctx = capi.Context(params(capi, capi.TT_PREC_FP64, 1, nlvls=1))
+    n = ctx.nel
+    x = np.random.default_rng(2024).uniform(0.0, 1.0, n)
+    xf = ctx.pde_filter(x)
+    xf = xf * (0.12 / xf.mean())
+    ctx.close()
+    return np.clip(xf, 0.001, 1.0)
+
+
+def op_microbench(capi, torch, nrep):
+    """1000 applies of K u, M u, A u at 128x64x64, FP64 and FP32; error of FP32 against FP64."""
+    ctx = capi.Context(params(capi, capi.TT_PREC_FP64, 200))
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0.001, 1.0, ctx.nel)
+    ctx.set_density(x)
+    nd, nel = ctx.nd, ctx.nel
+    u = rng.standard_normal(nd)
+    u.reshape(-1, 3)[0:(NEL[0] + 1) * 0] = 0.0
+    # zero the clamped dofs (x = 0 face)
+    u3 = u.reshape(NEL[2] + 1, NEL[1] + 1, NEL[0] + 1, 3)
+    u3[:, :, 0, :] = 0.0
+    dev = torch.device("cuda:0")
+    u64 = torch.from_numpy(u).to(dev)
+    u32 = u64.float()
+    y64 = torch.empty_like(u64)
+    y32 = torch.empty_like(u32)
+    out = {}
+    # minimum unique HBM traffic per apply: read u, write y, read kfac and mfac (LinearElasticity.cc:2502-2619)
+    for name, which in (("K", capi.TT_OP_K), ("M", capi.TT_OP_M), ("A", capi.TT_OP_A)):
+        ctx.op_apply(which, u64, y64, prec=0)
+        ctx.op_apply(which, u32, y32, prec=1)
+        ref = y64.double()
+        err = float(torch.linalg.norm(y32.double() - ref) / torch.linalg.norm(ref))
+        ms64 = ctx.time_op_apply(which, u64, y64, 0, nrep)
+        ms32 = ctx.time_op_apply(which, u32, y32, 1, nrep)
+        # algorithmic flops: K dense 24x24 per element; M as (8x8 scalar) x I3 per element
+        fl = 2 * nel * ((576 if which != capi.TT_OP_M else 0) + (192 if which != capi.TT_OP_K else 0))
+        b64 = 2 * nd * 8 + 2 * nel * 8
+        b32 = 2 * nd * 4 + 2 * nel * 4
+        out[name] = {
+            "fp64_ms": ms64, "fp32_ms": ms32, "fp64_applies_per_s": 1e3 / ms64, "fp32_applies_per_s": 1e3 / ms32,
+            "fp32_rel_err_vs_fp64": err, "flops": fl, "fp64_tflops": fl / (ms64 * 1e-3) / 1e12,
+            "fp32_tflops": fl / (ms32 * 1e-3) / 1e12, "fp64_GBs": b64 / (ms64 * 1e-3) / 1e9,
+            "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9,
+        }
+    ctx.close()
+    return out
+
+
+def forward_run(capi, x, precision, nsteps):
+    ctx = capi.Context(params(capi, precision, nsteps))
+    ctx.set_density(x)
+    ctx.complete_initial_state()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    ctx.forward()
+    el = time.perf_counter() - t0
+    st = ctx.stats()
+    u, _, _ = ctx.get_state(nsteps)
+    res = {"seconds": el, "iters_per_solve": st.krylov_iters / max(1, st.solves), "solves": st.solves,
+           "max_relres": st.max_relres, "lambda_max": list(st.lambda_max)[:st.nlvls]}
+    ctx.close()
+    return res, u
+
+
+def gradient_run(capi, x, precision, nsteps, slots):
+    ctx = capi.Context(params(capi, precision, nsteps, slots=slots))
+    ctx.set_density(x)
+    t0 = time.perf_counter()
+    ctx.complete_initial_state()
+    ctx.forward()
+    fx, df = ctx.objective_gradient()
+    el = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    return {"seconds": el, "forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
+            "solves": st.solves, "iters_per_solve": st.krylov_iters / max(1, st.solves)}, fx, df
+
+
+def cpu_baseline(steps):
+    """Oracle (FP64 C restatement, Jacobi-PCG) on the first `steps` Newmark steps of the same forward sweep."""
+    from oracle import oracle as O
+    x = None
+    p = O.cantilever_params(NEL[0], NEL[1], NEL[2], nsteps=steps, dt=DT, pulse_steps=20, rtol=1e-8)
+    n = O.nel(p)
+    x = np.full(n, 0.12)
+    t0 = time.perf_counter()
+    O.forward(p, x)
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": steps / el, "unit": "Newmark steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle forward, first {steps} steps of the 128x64x64 sweep (dt 0.005, rtol 1e-8), "
+                      f"uniform density 0.12, Jacobi-PCG, {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--slots", type=int, default=20)
+    ap.add_argument("--nrep", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--ops-only", action="store_true", help="only the operator microbenchmark (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU domain decomposition is a design note only).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from topopt_in_petsc_transient_amd import capi
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.set_device(0)
+    capi.lib()
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    log("density field")
+    x = density(capi)
+    log("operator microbenchmark")
+    ops = op_microbench(capi, torch, args.nrep)
+    if args.ops_only:
+        print(json.dumps({"operator": ops}))
+        return
+    log("warmup")
+    forward_run(capi, x, capi.TT_PREC_MIXED, max(1, args.warmup))
+    torch.cuda.synchronize()
+    log("timed mixed forward")
+    mixed, u_mixed = forward_run(capi, x, capi.TT_PREC_MIXED, args.steps)
+    torch.cuda.synchronize()
+    log("FP64 forward")
+    fp64, u_fp64 = forward_run(capi, x, capi.TT_PREC_FP64, args.steps)
+    u_err = float(np.linalg.norm(u_mixed - u_fp64) / max(np.linalg.norm(u_fp64), 1e-300))
+    grad = None
+    if not args.no_grad:
+        log("adjoint gradient, mixed")
+        gm, fxm, dfm = gradient_run(capi, x, capi.TT_PREC_MIXED, args.steps, args.slots)
+        log("adjoint gradient, FP64")
+        gd, fxd, dfd = gradient_run(capi, x, capi.TT_PREC_FP64, args.steps, args.slots)
+        grad = {"mixed": gm, "fp64": gd, "speedup": gd["seconds"] / gm["seconds"],
+                "fx_rel_err": abs(fxm - fxd) / abs(fxd),
+                "dfdx_rel_err": float(np.linalg.norm(dfm - dfd) / np.linalg.norm(dfd)), "fx": fxd,
+                "slots": args.slots}
+    cpu = None
+    if not args.no_cpu:
+        log("CPU oracle sample")
+        cpu = cpu_baseline(args.cpu_steps)
+
+    # roofline of the dominant kernel: the FP32 Newmark-operator apply of the V-cycle smoother
+    a32 = ops["A"]
+    ach = a32["flops"] / (a32["fp32_ms"] * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+            "frac": ach / (PEAK_FP32 / 1e12), "traffic": None,
+            "kernel": "k_apply<float,float,...> (A = K + a2 C + a5 M, FP32, 128x64x64)",
+            "kernel_ms": a32["fp32_ms"]}
+    value = args.steps / mixed["seconds"]
+    rec = {
+        "metric": METRIC, "value": value, "unit": "Newmark steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * mixed["seconds"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64 (mixed: FP32/BF16 multigrid, FP64 outer CG)",
+        "data": "synthetic (seeded filtered-noise density, half-sine line load)",
+        "config": {"workload": "128x64x64 hex8 cantilever, Newmark dt=0.005, 4-level... see nlvls",
+                   "nlvls": int(os.environ.get("TT_NLVLS", 5)), "rtol": 1e-8, "cheb_degree": 2,
+                   "parallelism": "single GPU"},
+        "fp64_steps_per_s": args.steps / fp64["seconds"], "mixed_vs_fp64_speedup": fp64["seconds"] / mixed["seconds"],
+        "u_final_rel_err_mixed_vs_fp64": u_err, "forward_mixed": mixed, "forward_fp64": fp64,
+        "adjoint_gradient": grad, "operator": ops, "roofline": roof, "cpu_baseline": cpu,
+    }
+    rec["config"]["workload"] = (f"128x64x64 hex8 cantilever (1,635,075 dofs), Newmark beta=1/4 dt={DT}, "
+                                 f"{rec['config']['nlvls']}-level MG Chebyshev(2) in FP64 FCG rtol 1e-8")
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

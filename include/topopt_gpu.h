@@ -149,6 +149,11 @@ int tt_mma_kkt_residual(tt_mma* mma, const double* x, const double* dfdx, const 
                         int on_device);                                                  /* MMA.cc:416-484 */
 int tt_mma_get_asymptotes(tt_mma* mma, double* L, double* U, double* lam); /* host arrays (nel, nel, 1) */
 
+/* Measurement helper with no counterpart in the reference: nrep back-to-back launches of the
+   tt_op_apply kernel on the context's stream, timed with HIP events on that stream; returns the
+   average milliseconds per launch.  u and y are device arrays (FP64 or FP32 per prec). */
+int tt_time_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int nrep, double* ms_per_apply);
+
 int tt_get_stats(tt_ctx* ctx, tt_stats* st);
 int tt_reset_stats(tt_ctx* ctx);
 int tt_synchronize(tt_ctx* ctx);

@@ -25,7 +25,7 @@ EXPORTED = [
     "tt_pde_filter", "tt_filter_project", "tt_filter_gradients", "tt_mnd", "tt_mma_create", "tt_mma_destroy",
     "tt_mma_set_asymptotes", "tt_mma_set_outer_movelimit", "tt_mma_update", "tt_mma_design_change",
     "tt_mma_kkt_residual", "tt_mma_get_asymptotes", "tt_get_stats", "tt_reset_stats", "tt_synchronize",
-    "tt_num_nodes", "tt_num_elements",
+    "tt_num_nodes", "tt_num_elements", "tt_time_op_apply",
 ]
 
 
@@ -90,6 +90,7 @@ def lib():
             "tt_mma_design_change": [vp, vp, vp, dp, ip],
             "tt_mma_kkt_residual": [vp, vp, vp, vp, vp, vp, vp, dp, dp, ip],
             "tt_mma_get_asymptotes": [vp, vp, vp, dp],
+            "tt_time_op_apply": [vp, ip, ip, vp, vp, ip, dp],
             "tt_get_stats": [vp, ctypes.POINTER(Stats)], "tt_reset_stats": [vp], "tt_synchronize": [vp],
         }
         for name, args in sig.items():
@@ -165,6 +166,13 @@ class Context:
         pu, dev = _ptr(u)
         py, _ = _ptr(y)
         self._chk(lib().tt_op_apply(self.h, which, prec, pu, py, dev))
+
+    def time_op_apply(self, which, u, y, prec, nrep):
+        ms = ctypes.c_double()
+        pu, _ = _ptr(u)
+        py, _ = _ptr(y)
+        self._chk(lib().tt_time_op_apply(self.h, which, prec, pu, py, nrep, ctypes.byref(ms)))
+        return ms.value
 
     def complete_initial_state(self):
         self._chk(lib().tt_complete_initial_state(self.h))

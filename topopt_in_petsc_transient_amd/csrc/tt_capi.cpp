@@ -28,7 +28,7 @@ Context::Context(const tt_params& p) : prm(p) {
     TT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     g = make_grid(p.nelx, p.nely, p.nelz, p.xc[1] - p.xc[0], p.xc[3] - p.xc[2], p.xc[5] - p.xc[4]);
     const long nd = g.nd, nn = g.nn, nel = g.nel;
-    const long apply_blocks = (long)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 3) / 4);
+    const long apply_blocks = 8 * (((long)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 1) / 2) + 7) / 8);
     red.init((int)std::max<long>({apply_blocks, (nel + 255) / 256, (nn + 255) / 256, 2048}));
 
     // Newmark constants (LinearElasticity.cc:185-207)
@@ -57,6 +57,15 @@ Context::Context(const tt_params& p) : prm(p) {
     TT_HIP(hipMemcpy(M864d.p, M8, sizeof(M8), hipMemcpyHostToDevice));
     TT_HIP(hipMemcpy(KE32d.p, ke32, sizeof(ke32), hipMemcpyHostToDevice));
     TT_HIP(hipMemcpy(M832d.p, m832, sizeof(m832), hipMemcpyHostToDevice));
+    {
+        double af64[18 * 64];
+        float af32[18 * 64];
+        mfma_fragments(KE, M8, af64, af32);
+        AF64d.alloc(18 * 64);
+        AF32d.alloc(18 * 64);
+        TT_HIP(hipMemcpy(AF64d.p, af64, sizeof(af64), hipMemcpyHostToDevice));
+        TT_HIP(hipMemcpy(AF32d.p, af32, sizeof(af32), hipMemcpyHostToDevice));
+    }
 
     // Dirichlet mask, load location, optimisation region (index form of the reference's
     // coordinate tests with tolerance 0.05 h, LinearElasticity.cc:222-335, 540-575, 616-675)
@@ -121,6 +130,8 @@ Context::Context(const tt_params& p) : prm(p) {
     op_fine64.M864 = M864d.p;
     op_fine64.KE32 = KE32d.p;
     op_fine64.M832 = M832d.p;
+    op_fine64.AF64 = AF64d.p;
+    op_fine64.AF32 = AF32d.p;
     op_fine64.mask = mask.p;
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
@@ -130,7 +141,7 @@ Context::Context(const tt_params& p) : prm(p) {
         b->alloc(nd);
     TT_HIP(hipMemsetAsync(lamT.p, 0, nd * 8, stream));
     TT_HIP(hipMemsetAsync(nu0.p, 0, nd * 8, stream));
-    fxs.alloc(p.nsteps + 1);
+    fxs.alloc((size_t)(p.nsteps + 1) * ((nel + 255) / 256));  // per-step objective partials
 
     // state storage: every state, or s binomial slots + the t = 0 state
     const int T = p.nsteps;
@@ -170,45 +181,41 @@ void Context::pde_filter(const double* in, double* out) {
     launch_filter_T(fg, in, vol, frhs.p, stream);  // RHS = elemVol * T x (:204-208)
     launch_filter_apply(fg, KFd.p, fU.p, fq.p, nullptr, 0, stream);
     launch_lincomb(1.0, frhs.p, -1.0, fq.p, fr.p, nn, stream);
-    launch_dot(frhs.p, frhs.p, nn, red, SC_TMP1, stream);
-    launch_dot(fr.p, fr.p, nn, red, SC_RR, stream);
-    TT_HIP(hipMemcpyAsync(red.host, red.scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, stream));
-    sync();
-    const double bn = std::sqrt(red.host[SC_TMP1]);
+    launch_dot(frhs.p, frhs.p, nn, red, PB_MISC, stream);
+    launch_dot(fr.p, fr.p, nn, red, PB_MISC2, stream);
+    const double bn = std::sqrt(reduce1(PB_MISC));
     if (!std::isfinite(bn)) throw Error{TT_ERR_NONFINITE, "non-finite PDE filter input"};
     const double tol = prm.filter_rtol;
-    double rel = bn > 0 ? std::sqrt(red.host[SC_RR]) / bn : 0.0;
+    double rel = bn > 0 ? std::sqrt(reduce1(PB_MISC2)) / bn : 0.0;
     if (bn == 0.0) launch_fill(fU.p, 0.0, nn, true, stream);
     int it = 0;
     const int maxit = 20000;
     while (bn > 0.0 && rel > tol && it < maxit) {
         launch_jacobi(fdinv.p, fr.p, fz.p, nn, stream);
         launch_lincomb(1.0, fz.p, 0.0, fz.p, fp.p, nn, stream);
-        launch_dot(fr.p, fz.p, nn, red, SC_GAMMA, stream);
-        double rz = read_scal(SC_GAMMA);
-        bool restart = false;
+        launch_dot(fr.p, fz.p, nn, red, PB_MISC, stream);
+        double rz = reduce1(PB_MISC);
         while (it < maxit) {
             it++;
-            launch_filter_apply(fg, KFd.p, fp.p, fq.p, &red, SC_PQ, stream);
-            const double alpha = rz / read_scal(SC_PQ);
+            launch_filter_apply(fg, KFd.p, fp.p, fq.p, &red, PB_PQ, stream);
+            const double alpha = rz / reduce1(PB_PQ);
             launch_lincomb(1.0, fU.p, alpha, fp.p, fU.p, nn, stream);
             launch_lincomb(1.0, fr.p, -alpha, fq.p, fr.p, nn, stream);
-            launch_dot(fr.p, fr.p, nn, red, SC_RR, stream);
-            rel = std::sqrt(read_scal(SC_RR)) / bn;
+            launch_dot(fr.p, fr.p, nn, red, PB_RR, stream);
+            rel = std::sqrt(reduce1(PB_RR)) / bn;
             if (!std::isfinite(rel)) throw Error{TT_ERR_NONFINITE, "non-finite PDE filter residual"};
             if (rel <= tol) break;
             launch_jacobi(fdinv.p, fr.p, fz.p, nn, stream);
-            launch_dot(fr.p, fz.p, nn, red, SC_GAMMA, stream);
-            const double rzn = read_scal(SC_GAMMA);
+            launch_dot(fr.p, fz.p, nn, red, PB_MISC, stream);
+            const double rzn = reduce1(PB_MISC);
             launch_lincomb(1.0, fz.p, rzn / rz, fp.p, fp.p, nn, stream);
             rz = rzn;
         }
         // true residual check
         launch_filter_apply(fg, KFd.p, fU.p, fq.p, nullptr, 0, stream);
         launch_lincomb(1.0, frhs.p, -1.0, fq.p, fr.p, nn, stream);
-        launch_dot(fr.p, fr.p, nn, red, SC_RR, stream);
-        rel = std::sqrt(read_scal(SC_RR)) / bn;
-        (void)restart;
+        launch_dot(fr.p, fr.p, nn, red, PB_RR, stream);
+        rel = std::sqrt(reduce1(PB_RR)) / bn;
     }
     if (rel > tol) throw Error{TT_ERR_NOT_CONVERGED, "PDE filter solve did not converge"};
     launch_filter_Tt(fg, fU.p, out, stream);  // T' U (:216)
@@ -252,12 +259,11 @@ static void mma_solve_dip(tt_mma* o, double* x) {
             loop++;
             xyz();
             launch_mma_xyz_grad_hess(o->dev(), o->lam, x, c->red, c->stream);
-            TT_HIP(hipMemcpyAsync(c->red.host + SC_TMP0, c->red.scal.p + SC_TMP0, 2 * sizeof(double),
-                                  hipMemcpyDeviceToHost, c->stream));
-            c->sync();
-            double grad = c->red.host[SC_TMP0] + (-o->b - o->a * o->z - o->y);  // DualGrad (:729-765)
+            double gh[4];
+            c->red.host_sum(PB_MISC, c->stream, gh);
+            double grad = gh[0] + (-o->b - o->a * o->z - o->y);  // DualGrad (:729-765)
             grad = -1.0 * grad - epsi / o->lam;
-            double H = c->red.host[SC_TMP0 + 1];  // DualHess (:767-871)
+            double H = gh[1];  // DualHess (:767-871)
             if (o->lam < 0.0) o->lam = 0.0;
             const double lamai = o->lam * o->a;
             if (o->lam > o->c) H += -1.0;
@@ -276,7 +282,7 @@ static void mma_solve_dip(tt_mma* o, double* x) {
             o->mu = o->mu + theta * s1;
             xyz();
             launch_mma_xyz_res(o->dev(), o->lam, x, c->red, c->stream);
-            const double rs = c->read_scal(SC_TMP0);
+            const double rs = c->reduce1(PB_MISC);
             const double r0 = rs + (-o->b - o->a * o->z - o->y + o->mu);  // DualResidual (:893-934)
             const double r1 = 0.0 + o->mu * o->lam - epsi;
             err = std::max(std::fabs(r0), std::fabs(r1));
@@ -409,8 +415,8 @@ int tt_set_density(tt_ctx* ctx, const double* xphys, int on_device) {
     const long nel = C.g.nel;
     if (on_device) tt::launch_copy(xphys, C.xphys.p, nel, true, true, C.stream);
     else TT_HIP(hipMemcpyAsync(C.xphys.p, xphys, nel * 8, hipMemcpyHostToDevice, C.stream));
-    tt::launch_sum(C.xphys.p, nel, C.red, tt::SC_TMP3, C.stream);
-    const double s = C.read_scal(tt::SC_TMP3);
+    tt::launch_sum(C.xphys.p, nel, C.red, tt::PB_MISC, C.stream);
+    const double s = C.reduce1(tt::PB_MISC);
     TT_CHECK(std::isfinite(s), TT_ERR_NONFINITE, "non-finite density");
     const tt_params& p = C.prm;
     tt::launch_interp(C.xphys.p, nel, p.sInt, p.Emin, p.Emax, p.penalK, p.mInt, p.Rmin, p.Rmax, p.penalM, C.P.p,
@@ -436,6 +442,7 @@ int tt_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int on
     tt::OpLevel op = C.fine_op(prec == 0);
     if (on_device) {
         tt::launch_apply_bc(op, cK, cM, u, y, prec == 0, C.stream);
+        C.sync();  // calls are synchronous at return
     } else {
         DBuf<char> du(nd * es), dy(nd * es);
         TT_HIP(hipMemcpyAsync(du.p, u, nd * es, hipMemcpyHostToDevice, C.stream));
@@ -443,6 +450,29 @@ int tt_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int on
         TT_HIP(hipMemcpyAsync(y, dy.p, nd * es, hipMemcpyDeviceToHost, C.stream));
         C.sync();
     }
+    TT_API_END
+}
+
+int tt_time_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int nrep, double* ms) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(C.density_set, TT_ERR_STATE, "tt_set_density must be called first");
+    TT_CHECK(u && y && ms && nrep > 0 && (prec == 0 || prec == 1) && which >= 0 && which <= 2, TT_ERR_ARG,
+             "bad tt_time_op_apply arguments");
+    double cK = which == TT_OP_K ? 1.0 : which == TT_OP_M ? 0.0 : C.cK_A;
+    double cM = which == TT_OP_K ? 0.0 : which == TT_OP_M ? 1.0 : C.cM_A;
+    tt::OpLevel op = C.fine_op(prec == 0);
+    hipEvent_t e0, e1;
+    TT_HIP(hipEventCreate(&e0));
+    TT_HIP(hipEventCreate(&e1));
+    TT_HIP(hipEventRecord(e0, C.stream));
+    for (int i = 0; i < nrep; i++) tt::launch_apply_bc(op, cK, cM, u, y, prec == 0, C.stream);
+    TT_HIP(hipEventRecord(e1, C.stream));
+    TT_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    TT_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / nrep;
     TT_API_END
 }
 
@@ -492,8 +522,8 @@ int tt_volume_constraint(tt_ctx* ctx, const double* xphys, double volfrac, doubl
     TT_CHECK(xphys && gx, TT_ERR_ARG, "null argument");
     const long nel = C.g.nel;
     DevIn xi(C, xphys, nel, on_device);
-    tt::launch_sum(xi.p, nel, C.red, tt::SC_TMP3, C.stream);
-    const double s = C.read_scal(tt::SC_TMP3);
+    tt::launch_sum(xi.p, nel, C.red, tt::PB_MISC, C.stream);
+    const double s = C.reduce1(tt::PB_MISC);
     *gx = s / (double)nel - volfrac;
     if (dgdx) {
         DevOut o(C, dgdx, nel, on_device);
@@ -553,8 +583,8 @@ int tt_mnd(tt_ctx* ctx, const double* x, double* mnd, int on_device) {
     TT_CHECK(x && mnd, TT_ERR_ARG, "null argument");
     const long nel = C.g.nel;
     DevIn xi(C, x, nel, on_device);
-    tt::launch_mnd(xi.p, nel, C.red, tt::SC_TMP3, C.stream);
-    *mnd = C.read_scal(tt::SC_TMP3) / (double)nel;
+    tt::launch_mnd(xi.p, nel, C.red, tt::PB_MISC, C.stream);
+    *mnd = C.reduce1(tt::PB_MISC) / (double)nel;
     TT_API_END
 }
 
@@ -613,7 +643,7 @@ int tt_mma_update(tt_mma* m, double* x, const double* dfdx, const double* gx, co
     m->k++;  // GenSub (MMA.cc:487-600)
     tt::launch_mma_gensub(m->dev(), xo.p, df.p, dg.p, lo.p, hi.p, m->k, m->asyminit, m->asymdec, m->asyminc, C.red,
                           C.stream);
-    m->b = C.read_scal(tt::SC_TMP0) + (-gx[0]);
+    m->b = C.reduce1(tt::PB_MISC) + (-gx[0]);
     tt::mma_solve_dip(m, xo.p);
     xo.finish();
     TT_API_END
@@ -638,9 +668,7 @@ int tt_mma_kkt_residual(tt_mma* m, const double* x, const double* dfdx, const do
     DevIn xi(C, x, n, on_device), df(C, dfdx, n, on_device), dg(C, dgdx, n, on_device), lo(C, xmin, n, on_device),
         hi(C, xmax, n, on_device);
     tt::launch_mma_kkt(xi.p, df.p, dg.p, lo.p, hi.p, n, m->lam, C.red, C.stream);
-    TT_HIP(hipMemcpyAsync(C.red.host, C.red.scal.p, sizeof(double) * tt::SC_COUNT, hipMemcpyDeviceToHost, C.stream));
-    C.sync();
-    double n2 = C.red.host[tt::SC_TMP1], ni = C.red.host[tt::SC_TMP2];
+    double n2 = C.reduce1(tt::PB_MISC), ni = C.read_scal(tt::SC_TMP2);
     const double ri = m->lam * (m->a * m->z + m->y - gx[0]);  // MMA.cc:476-482
     n2 += std::pow(ri, 2.0);
     ni = std::max(std::fabs(ri), ni);

@@ -20,8 +20,8 @@ struct MGLevel {
     DBuf<double> kf64, mf64;           // FP64 coefficients (coarse levels: averages of children)
     DBuf<float> kf32, mf32;            // mixed mode, finest level
     DBuf<bf16_t> kfbf, mfbf;           // mixed mode, coarse levels
-    DBuf<double> KE64, M864;
-    DBuf<float> KE32, M832;
+    DBuf<double> KE64, M864, AF64;
+    DBuf<float> KE32, M832, AF32;
     OpLevel op{};
     DBuf<char> b, x, x2, r, d, d2, dinv;  // level vectors (element size = 8 or 4 bytes)
     double lmax = 1.0;
@@ -82,8 +82,8 @@ struct Context {
     DBuf<uint8_t> mask;
     std::vector<double> Fhost;
     OpLevel op_fine64{};  // FP64 fine operator view (for FCG residuals)
-    DBuf<double> KE64d, M864d;
-    DBuf<float> KE32d, M832d;
+    DBuf<double> KE64d, M864d, AF64d;
+    DBuf<float> KE32d, M832d, AF32d;
     DBuf<float> kf32, mf32;  // FP32 fine coefficients for the FP32 microbenchmark apply
     bool density_set = false;
     bool initial_done = false;
@@ -121,6 +121,7 @@ struct Context {
     SolveResult fcg(double cK, double cM, bool use_mg, const double* b, double* x);
     void sync();
     double read_scal(int slot);
+    double reduce1(int region);  // host sum of a partial region (synchronises)
 
     // transient
     void complete_initial_state();

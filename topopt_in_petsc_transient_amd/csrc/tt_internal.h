@@ -92,15 +92,24 @@ void hex8_mass(double hx, double hy, double hz, double* m8);                 // 
 void hex8_helmholtz(double hx, double hy, double hz, double R, double* kf);  // 8x8, R^2 Lap + mass
 
 // ------------------------------------------------------------------ reductions
-// Deterministic device reductions: every reducing kernel writes one partial per
-// block, and the last block to finish sums the partials in block order.
+// Deterministic device reductions without fences or atomics: a reducing kernel
+// writes one partial per block into a partial buffer; the next kernel on the
+// stream (or the host) sums the partials in block order.  Kernel boundaries
+// give the visibility, so no block pays a device-scope fence (which on a
+// multi-XCD GPU means an L2 write-back).
+enum { PB_DOTS = 0, PB_PQ, PB_RR, PB_MISC, PB_MISC2, PB_COUNT };
 struct Reducer {
-    DBuf<double> partials;   // [maxblocks * 8]
-    DBuf<unsigned> counter;  // [8]
+    DBuf<double> partials;   // PB_COUNT regions of [maxblocks * 4]
     DBuf<double> scal;       // device scalars (see SC_* indices)
     double* host = nullptr;  // pinned mirror of scal
+    double* hpart = nullptr; // pinned host copy of one partial region
     int maxblocks = 0;
+    int nblk[PB_COUNT] = {};  // blocks that wrote each region
+    int nv[PB_COUNT] = {};
     void init(int maxb);
+    double* part(int region) const { return partials.p + (size_t)region * maxblocks * 4; }
+    // sum a region on the host in block order (synchronises the stream)
+    void host_sum(int region, hipStream_t s, double* out);
     ~Reducer();
 };
 
@@ -134,9 +143,15 @@ struct OpLevel {
     const float* KE32;
     const float* M832;
     const uint8_t* mask;   // per node clamped bits
+    const double* AF64;    // MFMA A fragments of [KE; M8 (x) I3] in lane order (18 x 64)
+    const float* AF32;
 };
 
-// y = N.(cK*K + cM*M)(N.u) on free dofs, identity on clamped dofs.  Optional dot u.y into scal[dot_slot].
+// MFMA A fragments of the interleaved [KE; M8 (x) I3] matrix, lane order (see tt_kernels.hip)
+void mfma_fragments(const double* KE, const double* M8, double* af64, float* af32);
+
+// y = N.(cK*K + cM*M)(N.u) on free dofs, identity on clamped dofs.  With red: partials of u.y in
+// partial region dot_slot (PB_*).  Reductions below likewise take a partial region.
 void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void* y, bool fp64, hipStream_t s,
                      Reducer* red = nullptr, int dot_slot = -1);
 // r = b - A x
@@ -184,8 +199,9 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        hipStream_t s);
 void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipStream_t s);
+// power iteration: x = dinv.Ax with partials of ||x||^2 in PB_MISC; then x /= ||x||
 void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s);
-void launch_scale_by_scal(void* x, long n, const double* scal, int slot, bool fp64, hipStream_t s);
+void launch_scale_by_norm(void* x, long n, Reducer& red, bool fp64, hipStream_t s);
 
 // Newmark corrector: vn = a0 v + a1 a + a2 (un - u); an = -a3 v - a4 a + a5 (un - u)
 void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,

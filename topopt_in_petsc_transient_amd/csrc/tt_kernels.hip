@@ -53,60 +53,63 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
     }
 }
 
-// Writes the block's NV partials; the last block sums all partials in block
-// order and hands the totals to fin(totals) (thread 0 only).
-template <int NV, class Fin>
-__device__ __forceinline__ void reduce_finish(double (&v)[NV], double* partials, unsigned* counter, Fin fin) {
+// Writes the block's NV partial sums to part[block*NV + k] (plain stores).
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* part) {
     __shared__ double sh[16 * NV];
-    __shared__ bool last;
     block_sum<NV>(v, sh);
-    const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     if (threadIdx.x == 0) {
+        const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
-        for (int k = 0; k < NV; k++) partials[(size_t)bid * NV + k] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        last = (t == nblk - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < nblk; b += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-            acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    block_sum<NV>(acc, sh);
-    if (threadIdx.x == 0) {
-        fin(acc);
-        *counter = 0u;
+        for (int k = 0; k < NV; k++) part[(size_t)bid * NV + k] = v[k];
     }
 }
 
-struct FinStore1 {
-    double* out;
-    __device__ void operator()(const double (&t)[1]) const { out[0] = t[0]; }
-};
-struct FinStore2 {
-    double* out;
-    __device__ void operator()(const double (&t)[2]) const { out[0] = t[0]; out[1] = t[1]; }
-};
+// Sum of partials written by an earlier kernel, in block order; every thread gets the totals.
+// One wave reduces: lane l sums blocks l, l+64, ... then a fixed butterfly.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nblk, double (&out)[NV]) {
+    __shared__ double res[NV];
+    if (threadIdx.x < 64) {
+        double acc[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) acc[k] = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += 64)
+#pragma unroll
+            for (int k = 0; k < NV; k++) acc[k] += part[(size_t)b * NV + k];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double x = acc[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if (threadIdx.x == 0) res[k] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = res[k];
+}
 
 void Reducer::init(int maxb) {
     maxblocks = maxb;
-    partials.alloc((size_t)maxb * 8);
-    counter.alloc(8);
+    partials.alloc((size_t)PB_COUNT * maxb * 4);
     scal.alloc(SC_COUNT);
-    TT_HIP(hipMemset(counter.p, 0, sizeof(unsigned) * 8));
+    TT_HIP(hipMemset(partials.p, 0, sizeof(double) * PB_COUNT * maxb * 4));
     TT_HIP(hipMemset(scal.p, 0, sizeof(double) * SC_COUNT));
     TT_HIP(hipHostMalloc((void**)&host, sizeof(double) * SC_COUNT, hipHostMallocDefault));
+    TT_HIP(hipHostMalloc((void**)&hpart, sizeof(double) * maxb * 4, hipHostMallocDefault));
+}
+void Reducer::host_sum(int region, hipStream_t s, double* out) {
+    const int nb = nblk[region], v = nv[region];
+    TT_HIP(hipMemcpyAsync(hpart, part(region), sizeof(double) * nb * v, hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < v; k++) out[k] = 0.0;
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < v; k++) out[k] += hpart[(size_t)b * v + k];
 }
 Reducer::~Reducer() {
     if (host) (void)hipHostFree(host);
+    if (hpart) (void)hipHostFree(hpart);
 }
 
 static inline int nblocks(long n, int per = BS) { return (int)((n + per - 1) / per); }
@@ -174,60 +177,105 @@ __device__ __forceinline__ void node_contract(const T* __restrict__ sU, const T*
 template <typename T>
 struct StageOne {  // w = u
     const T* u;
-    __device__ __forceinline__ T k(long i) const { return u[i]; }
-    __device__ __forceinline__ T m(long i) const { return u[i]; }
+    __device__ __forceinline__ void prologue() {}
+    __device__ __forceinline__ T k(int i) const { return u[i]; }
+    __device__ __forceinline__ T m(int i) const { return u[i]; }
 };
+// Flexible-CG direction p = z + beta p_old with beta = z.(r - r_prev) / gamma_old, the two
+// dots summed from the partials of the preceding k_fcg_dots.
 template <typename T>
-struct StageFcg {  // p = z + beta*p_old, beta read from device scalar
+struct StageFcg {
     const T* z;
     const T* pold;
-    const double* beta;  // null on the first iteration
-    __device__ __forceinline__ T k(long i) const { return beta ? z[i] + (T)(*beta) * pold[i] : z[i]; }
-    __device__ __forceinline__ T m(long i) const { return k(i); }
+    const double* dots_part;
+    int dots_nblk;
+    const double* scal;
+    int first;
+    T beta;
+    __device__ __forceinline__ void prologue() {
+        if (first) {
+            beta = 0;
+            return;
+        }
+        double d[2];
+        sum_partials<2>(dots_part, dots_nblk, d);
+        beta = (T)((d[0] - d[1]) / scal[SC_GAMMA_OLD]);
+    }
+    __device__ __forceinline__ T k(int i) const { return first ? z[i] : z[i] + beta * pold[i]; }
+    __device__ __forceinline__ T m(int i) const { return k(i); }
 };
 struct StageThree {  // wK = sum_i cK_i in_i, wM = sum_i cM_i in_i (FP64)
     const double *u, *v, *a;
     double k0, k1, k2, m0, m1, m2;
-    __device__ __forceinline__ double k(long i) const { return k0 * u[i] + k1 * v[i] + k2 * a[i]; }
-    __device__ __forceinline__ double m(long i) const { return m0 * u[i] + m1 * v[i] + m2 * a[i]; }
+    __device__ __forceinline__ void prologue() {}
+    __device__ __forceinline__ double k(int i) const { return k0 * u[i] + k1 * v[i] + k2 * a[i]; }
+    __device__ __forceinline__ double m(int i) const { return m0 * u[i] + m1 * v[i] + m2 * a[i]; }
 };
 
 template <typename T, typename CT, bool TWO, bool MASKIN, class Stage, class Epi>
 __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ KE, const T* __restrict__ M8,
                                               const CT* __restrict__ kf, const CT* __restrict__ mf, T cK, T cM,
-                                              const uint8_t* __restrict__ mask, Stage st, Epi epi) {
+                                              const uint8_t* __restrict__ mask, Stage st_in, Epi epi) {
     __shared__ T sU[HN * 3];
     __shared__ T sW[TWO ? HN * 3 : 1];
     __shared__ T sk[EN], sm[EN];
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, k0 = blockIdx.z * TZ;
     const int tid = threadIdx.x;
-    for (int v = tid; v < HN * 3; v += NT) {
+    Stage st = st_in;
+    st.prologue();
+    // Stage the node box and element coefficients.  Every global load of the
+    // block is issued before the first LDS store, so the block waits for one
+    // memory latency instead of one per loop trip.
+    constexpr int NSV = (HN * 3 + NT - 1) / NT;
+    constexpr int NSE = (EN + NT - 1) / NT;
+    T val[NSV], valw[TWO ? NSV : 1];
+    T ca[NSE], cb[NSE];
+#pragma unroll
+    for (int q = 0; q < NSV; q++) {
+        const int v = tid + q * NT;
         const int node = v / 3, c = v - 3 * node;
         const int hx = node % HX, hy = (node / HX) % HY, hz = node / (HX * HY);
         const int gi = i0 - 1 + hx, gj = j0 - 1 + hy, gk = k0 - 1 + hz;
-        T val = 0, valw = 0;
-        if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) {
-            const long n = gi + (long)g.nx * (gj + (long)g.ny * gk);
-            const bool clamped = MASKIN && ((mask[n] >> c) & 1);
-            if (!clamped) {
-                val = st.k(3 * n + c);
-                if (TWO) valw = st.m(3 * n + c);
+        const bool in = v < HN * 3 && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz;
+        const long n = in ? gi + (long)g.nx * (gj + (long)g.ny * gk) : 0;
+        T x = in ? st.k(3 * n + c) : (T)0;
+        T w = 0;
+        if (TWO) w = in ? st.m(3 * n + c) : (T)0;
+        if (MASKIN) {
+            const uint8_t cl = in ? mask[n] : (uint8_t)0;
+            if ((cl >> c) & 1) {
+                x = 0;
+                w = 0;
             }
         }
-        sU[v] = val;
-        if (TWO) sW[v] = valw;
+        val[q] = x;
+        if (TWO) valw[q] = w;
     }
-    for (int e = tid; e < EN; e += NT) {
+#pragma unroll
+    for (int q = 0; q < NSE; q++) {
+        const int e = tid + q * NT;
         const int hx = e % EXs, hy = (e / EXs) % EYs, hz = e / (EXs * EYs);
         const int gi = i0 - 1 + hx, gj = j0 - 1 + hy, gk = k0 - 1 + hz;
-        T a = 0, b = 0;
-        if (gi >= 0 && gi < g.ex && gj >= 0 && gj < g.ey && gk >= 0 && gk < g.ez) {
-            const long ge = gi + (long)g.ex * (gj + (long)g.ey * gk);
-            a = cK * (T)coef_ld(kf, ge);
-            b = cM * (T)coef_ld(mf, ge);
+        const bool in = e < EN && gi >= 0 && gi < g.ex && gj >= 0 && gj < g.ey && gk >= 0 && gk < g.ez;
+        const long ge = in ? gi + (long)g.ex * (gj + (long)g.ey * gk) : 0;
+        ca[q] = in ? cK * (T)coef_ld(kf, ge) : (T)0;
+        cb[q] = in ? cM * (T)coef_ld(mf, ge) : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < NSV; q++) {
+        const int v = tid + q * NT;
+        if (v < HN * 3) {
+            sU[v] = val[q];
+            if (TWO) sW[v] = valw[q];
         }
-        sk[e] = a;
-        sm[e] = b;
+    }
+#pragma unroll
+    for (int q = 0; q < NSE; q++) {
+        const int e = tid + q * NT;
+        if (e < EN) {
+            sk[e] = ca[q];
+            sm[e] = cb[q];
+        }
     }
     __syncthreads();
     const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
@@ -244,12 +292,14 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
     epi(inside, n, cl, yK, yM, st);
 }
 
+#include "tt_apply_mfma.inc"
+
 // ---- epilogues.  All are called by every thread (inside or not) so that reducing epilogues can sync.
 template <typename T>
 struct EpiBC {  // y = A u on free dofs, u on clamped
     T* y;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
         if (!in) return;
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
@@ -257,13 +307,11 @@ struct EpiBC {  // y = A u on free dofs, u on clamped
 };
 
 template <typename T>
-struct EpiBCDot {  // y = A u (BC form) and sum u.y -> scal
+struct EpiBCDot {  // y = A u (BC form) and partials of u.y
     T* y;
-    double* partials;
-    unsigned* counter;
-    double* out;
+    double* part;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
         double acc[1] = {0.0};
         if (in) {
 #pragma unroll
@@ -274,7 +322,7 @@ struct EpiBCDot {  // y = A u (BC form) and sum u.y -> scal
                 acc[0] += (double)u * (double)v;
             }
         }
-        reduce_finish<1>(acc, partials, counter, FinStore1{out});
+        write_partials<1>(acc, part);
     }
 };
 
@@ -283,7 +331,7 @@ struct EpiResid {  // r = b - A x
     const T* b;
     T* r;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
         if (!in) return;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -302,11 +350,11 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
     T* xn;
     T* dn;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
+    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
         if (!in) return;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const long i = 3 * n + c;
+            const int i = 3 * n + c;
             const T x = st.k(i);
             const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
             const T r = b[i] - ax;
@@ -322,7 +370,7 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
     double* yk;
     double* ym;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const double (&yK)[3], const double (&yM)[3],
+    __device__ void operator()(bool in, int n,uint8_t cl, const double (&yK)[3], const double (&yM)[3],
                                const S&) const {
         if (!in) return;
 #pragma unroll
@@ -337,7 +385,7 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
 struct EpiRaw {  // y = yK + yM, no BC
     double* y;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t, const double (&yK)[3], const double (&yM)[3],
+    __device__ void operator()(bool in, int n,uint8_t, const double (&yK)[3], const double (&yM)[3],
                                const S&) const {
         if (!in) return;
 #pragma unroll
@@ -350,12 +398,12 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     double fs;
     double* R;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const double (&yK)[3], const double (&yM)[3],
+    __device__ void operator()(bool in, int n,uint8_t cl, const double (&yK)[3], const double (&yM)[3],
                                const S&) const {
         if (!in) return;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const long i = 3 * n + c;
+            const int i = 3 * n + c;
             double r = fs * F[i];
             r += yK[c];
             r += yM[c];
@@ -364,28 +412,18 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     }
 };
 
-struct FinAlpha {  // pq -> alpha = gamma / pq
-    double* scal;
-    __device__ void operator()(const double (&t)[1]) const {
-        scal[SC_PQ] = t[0];
-        scal[SC_ALPHA] = scal[SC_GAMMA] / t[0];
-    }
-};
-
-struct EpiFcg {  // p_new = staged p (owned node), q = A p, partial p.q
+struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
     double* pnew;
     double* q;
-    double* partials;
-    unsigned* counter;
-    double* scal;
+    double* part;
     template <class S>
-    __device__ void operator()(bool in, long n, uint8_t cl, const double (&yK)[3], const double (&yM)[3],
+    __device__ void operator()(bool in, int n,uint8_t cl, const double (&yK)[3], const double (&yM)[3],
                                const S& st) const {
         double acc[1] = {0.0};
         if (in) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const long i = 3 * n + c;
+                const int i = 3 * n + c;
                 const double p = st.k(i);
                 const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
                 pnew[i] = p;
@@ -393,7 +431,7 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partial p.q
                 acc[0] += p * v;
             }
         }
-        reduce_finish<1>(acc, partials, counter, FinAlpha{scal});
+        write_partials<1>(acc, part);
     }
 };
 
@@ -401,13 +439,52 @@ static inline dim3 apply_grid(const Grid& g) {
     return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + TZ - 1) / TZ);
 }
 
+// operator kernel family: 1 = element GEMM on the matrix cores (default), 0 = per-node vector FMA
+static int apply_impl() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("TT_APPLY_IMPL");
+        v = e ? atoi(e) : 1;
+    }
+    return v;
+}
+
 // dispatch on coefficient kind and precision
 template <typename T, bool TWO, bool MASKIN, class Stage, class Epi>
-static void run_apply(const OpLevel& L, double cK, double cM, const Stage& st, const Epi& epi, hipStream_t s) {
+static int run_apply(const OpLevel& L, double cK, double cM, const Stage& st, const Epi& epi, hipStream_t s,
+                     bool allow_mfma = true) {
     const T* KE = std::is_same<T, double>::value ? (const T*)L.KE64 : (const T*)L.KE32;
     const T* M8 = std::is_same<T, double>::value ? (const T*)L.M864 : (const T*)L.M832;
-    dim3 grid = apply_grid(L.g);
     GridArgs ga = gargs(L.g);
+    if (!TWO && allow_mfma && apply_impl() == 1) {
+        // FP32: 8x8x4 node tiles (50 KB LDS); FP64: 8x8x2 tiles keep the block under 64 KB of LDS
+        constexpr int TZm = std::is_same<T, double>::value ? 2 : 4;
+        typedef MTile<TZm> Tl;
+        const T* AF = std::is_same<T, double>::value ? (const T*)L.AF64 : (const T*)L.AF32;
+        const int tx = (L.g.nx + Tl::X - 1) / Tl::X, ty = (L.g.ny + Tl::Y - 1) / Tl::Y;
+        const int ntiles = tx * ty * ((L.g.nz + Tl::Z - 1) / Tl::Z);
+        dim3 grid(8 * ((ntiles + 7) / 8));  // 1-D, XCD-aware tile order inside the kernel
+        switch (L.coef_kind) {
+        case COEF_F64:
+            hipLaunchKernelGGL((k_apply_mfma<T, double, TZm, MASKIN, Stage, Epi>), grid, dim3(Tl::NT), 0, s, ga, tx,
+                               ty, ntiles, AF, (const double*)L.kf, (const double*)L.mf, (T)cK, (T)cM, L.mask, st,
+                               epi);
+            break;
+        case COEF_F32:
+            hipLaunchKernelGGL((k_apply_mfma<T, float, TZm, MASKIN, Stage, Epi>), grid, dim3(Tl::NT), 0, s, ga, tx,
+                               ty, ntiles, AF, (const float*)L.kf, (const float*)L.mf, (T)cK, (T)cM, L.mask, st,
+                               epi);
+            break;
+        default:
+            hipLaunchKernelGGL((k_apply_mfma<T, bf16_t, TZm, MASKIN, Stage, Epi>), grid, dim3(Tl::NT), 0, s, ga, tx,
+                               ty, ntiles, AF, (const bf16_t*)L.kf, (const bf16_t*)L.mf, (T)cK, (T)cM, L.mask, st,
+                               epi);
+            break;
+        }
+        TT_HIP(hipGetLastError());
+        return (int)grid.x;
+    }
+    dim3 grid = apply_grid(L.g);
     switch (L.coef_kind) {
     case COEF_F64:
         hipLaunchKernelGGL((k_apply<T, double, TWO, MASKIN, Stage, Epi>), grid, dim3(NT), 0, s, ga, KE, M8,
@@ -423,11 +500,12 @@ static void run_apply(const OpLevel& L, double cK, double cM, const Stage& st, c
         break;
     }
     TT_HIP(hipGetLastError());
+    return (int)(grid.x * grid.y * grid.z);
 }
 
+// upper bound of the blocks of any apply launch on grid g (8x8x2 tiles, rounded up to a multiple of 8)
 static inline int apply_nblocks(const Grid& g) {
-    dim3 d = apply_grid(g);
-    return (int)(d.x * d.y * d.z);
+    return 8 * ((((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 1) / 2) + 7) / 8);
 }
 
 void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void* y, bool fp64, hipStream_t s,
@@ -436,8 +514,9 @@ void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void
         StageOne<double> st{(const double*)u};
         if (red) {
             TT_CHECK(apply_nblocks(L.g) <= red->maxblocks, -3, "reducer too small");
-            EpiBCDot<double> e{(double*)y, red->partials.p, red->counter.p, red->scal.p + dot_slot};
-            run_apply<double, false, true>(L, cK, cM, st, e, s);
+            EpiBCDot<double> e{(double*)y, red->part(dot_slot)};
+            red->nblk[dot_slot] = run_apply<double, false, true>(L, cK, cM, st, e, s);
+            red->nv[dot_slot] = 1;
         } else {
             run_apply<double, false, true>(L, cK, cM, st, EpiBC<double>{(double*)y}, s);
         }
@@ -445,8 +524,9 @@ void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void
         StageOne<float> st{(const float*)u};
         if (red) {
             TT_CHECK(apply_nblocks(L.g) <= red->maxblocks, -3, "reducer too small");
-            EpiBCDot<float> e{(float*)y, red->partials.p, red->counter.p, red->scal.p + dot_slot};
-            run_apply<float, false, true>(L, cK, cM, st, e, s);
+            EpiBCDot<float> e{(float*)y, red->part(dot_slot)};
+            red->nblk[dot_slot] = run_apply<float, false, true>(L, cK, cM, st, e, s);
+            red->nv[dot_slot] = 1;
         } else {
             run_apply<float, false, true>(L, cK, cM, st, EpiBC<float>{(float*)y}, s);
         }
@@ -477,7 +557,7 @@ void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, con
 
 void launch_apply_split(const OpLevel& L, double cK, double cM, const double* u, double* yk, double* ym,
                         hipStream_t s) {
-    run_apply<double, false, true>(L, cK, cM, StageOne<double>{u}, EpiSplit{yk, ym}, s);
+    run_apply<double, false, true>(L, cK, cM, StageOne<double>{u}, EpiSplit{yk, ym}, s, false);
 }
 
 void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, double cK, double cM, const double* u,
@@ -498,9 +578,10 @@ void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, cons
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
                       double* q, Reducer& red, bool first, hipStream_t s) {
     TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
-    StageFcg<double> st{z, p_old, first ? nullptr : red.scal.p + SC_BETA};
-    EpiFcg e{p_new, q, red.partials.p, red.counter.p, red.scal.p};
-    run_apply<double, false, true>(L, cK, cM, st, e, s);
+    StageFcg<double> st{z, p_old, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p, first ? 1 : 0, 0.0};
+    EpiFcg e{p_new, q, red.part(PB_PQ)};
+    red.nblk[PB_PQ] = run_apply<double, false, true>(L, cK, cM, st, e, s);
+    red.nv[PB_PQ] = 1;
 }
 
 // ------------------------------------------------------------------ diagonal
@@ -753,64 +834,81 @@ void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, 
     TT_HIP(hipGetLastError());
 }
 
-// dense coarsest-level solve x = Ainv b (Ainv column-major, n x n)
+// dense coarsest-level solve x = Ainv b (Ainv symmetric, row-major n x n): one wave per row,
+// lanes stride along the row (coalesced), fixed butterfly sum
 __global__ void k_dense_gemv(const double* __restrict__ A, const double* __restrict__ b, double* __restrict__ x,
                              int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* Ar = A + (size_t)row * n;
     double s = 0.0;
-    for (int j = 0; j < n; j++) s = fma(A[(size_t)j * n + i], b[j], s);
-    x[i] = s;
+    for (int j = lane; j < n; j += 64) s = fma(Ar[j], b[j], s);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) x[row] = s;
 }
 void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_dense_gemv, dim3(nblocks(n, 64)), dim3(64), 0, s, Ainv, b, x, n);
+    hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4), dim3(256), 0, s, Ainv, b, x, n);
     TT_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ reductions / Krylov
-__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n, double* partials,
-                      unsigned* counter, double* out) {
+// Reductions over long vectors use at most RB blocks (grid-stride), so that the
+// consumers of their partials (next kernel or host) sum few values.
+constexpr int RB = 256;
+static inline int rb_blocks(long n) {
+    int b = nblocks(n);
+    return b > RB ? RB : (b < 1 ? 1 : b);
+}
+
+__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n, double* part) {
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         acc[0] = fma(a[i], b[i], acc[0]);
-    reduce_finish<1>(acc, partials, counter, FinStore1{out});
+    write_partials<1>(acc, part);
 }
-void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(BS), 0, s, a, b, n, red.partials.p, red.counter.p,
-                       red.scal.p + slot);
+void launch_dot(const double* a, const double* b, long n, Reducer& red, int region, hipStream_t s) {
+    const int nb = rb_blocks(n);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(BS), 0, s, a, b, n, red.part(region));
+    red.nblk[region] = nb;
+    red.nv[region] = 1;
     TT_HIP(hipGetLastError());
 }
 
-struct FinFcgDots {  // gamma = z.r, delta = z.r_prev; beta = (gamma - delta)/gamma_old
-    double* scal;
-    bool first;
-    __device__ void operator()(const double (&t)[2]) const {
-        scal[SC_BETA] = first ? 0.0 : (t[0] - t[1]) / scal[SC_GAMMA_OLD];
-        scal[SC_GAMMA] = t[0];
-        scal[SC_DELTA] = t[1];
-        scal[SC_GAMMA_OLD] = t[0];
-    }
-};
+// gamma = z.r, delta = z.r_prev (flexible CG, Polak-Ribiere form)
 __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ rp,
-                           long n, double* partials, unsigned* counter, double* scal, bool first) {
+                           long n, double* part, int first) {
     double acc[2] = {0.0, 0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         acc[0] = fma(z[i], r[i], acc[0]);
         if (!first) acc[1] = fma(z[i], rp[i], acc[1]);
     }
-    reduce_finish<2>(acc, partials, counter, FinFcgDots{scal, first});
+    write_partials<2>(acc, part);
 }
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_dots, dim3(red_blocks(n)), dim3(BS), 0, s, z, r, rprev, n, red.partials.p,
-                       red.counter.p, red.scal.p, first);
+    const int nb = rb_blocks(n);
+    hipLaunchKernelGGL(k_fcg_dots, dim3(nb), dim3(BS), 0, s, z, r, rprev, n, red.part(PB_DOTS), (int)first);
+    red.nblk[PB_DOTS] = nb;
+    red.nv[PB_DOTS] = 2;
     TT_HIP(hipGetLastError());
 }
 
+// x += alpha p; r_prev = r; r -= alpha q with alpha = gamma / (p.q) summed from the partials of
+// k_fcg_dots and of the FCG apply; partials of r.r.  Block 0 keeps gamma for the next beta.
 __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ p, double* __restrict__ r,
-                             double* __restrict__ rp, const double* __restrict__ q, long n, double* partials,
-                             unsigned* counter, double* scal) {
-    const double al = scal[SC_ALPHA];
+                             double* __restrict__ rp, const double* __restrict__ q, long n,
+                             const double* __restrict__ dots_part, int dots_nblk, const double* __restrict__ pq_part,
+                             int pq_nblk, double* part, double* scal) {
+    double d[2], pq[1];
+    sum_partials<2>(dots_part, dots_nblk, d);
+    sum_partials<1>(pq_part, pq_nblk, pq);
+    const double al = d[0] / pq[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[SC_GAMMA_OLD] = d[0];
+        scal[SC_ALPHA] = al;
+    }
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         x[i] = fma(al, p[i], x[i]);
@@ -820,12 +918,15 @@ __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ 
         r[i] = rn;
         acc[0] = fma(rn, rn, acc[0]);
     }
-    reduce_finish<1>(acc, partials, counter, FinStore1{scal + SC_RR});
+    write_partials<1>(acc, part);
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_update, dim3(red_blocks(n)), dim3(BS), 0, s, x, p, r, rprev, q, n, red.partials.p,
-                       red.counter.p, red.scal.p);
+    const int nb = rb_blocks(n);
+    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.part(PB_DOTS),
+                       red.nblk[PB_DOTS], red.part(PB_PQ), red.nblk[PB_PQ], red.part(PB_RR), red.scal.p);
+    red.nblk[PB_RR] = nb;
+    red.nv[PB_RR] = 1;
     TT_HIP(hipGetLastError());
 }
 
@@ -839,36 +940,48 @@ void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipSt
     TT_HIP(hipGetLastError());
 }
 
-// power iteration helper: x = dinv.Ax, and ||x||^2 -> scal[SC_TMP0]
+// power iteration helper: x = dinv.Ax, partials of ||x||^2 in PB_MISC
 template <typename T>
 __global__ void k_norm_power(const T* __restrict__ Ax, const T* __restrict__ dinv, T* __restrict__ x, long n,
-                             double* partials, unsigned* counter, double* out) {
+                             double* part) {
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const T v = dinv[i] * Ax[i];
         x[i] = v;
         acc[0] += (double)v * (double)v;
     }
-    reduce_finish<1>(acc, partials, counter, FinStore1{out});
+    write_partials<1>(acc, part);
 }
 void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s) {
+    const int nb = rb_blocks(n);
     if (fp64)
-        hipLaunchKernelGGL(k_norm_power<double>, dim3(red_blocks(n)), dim3(BS), 0, s, (const double*)Ax,
-                           (const double*)dinv, (double*)x, n, red.partials.p, red.counter.p, red.scal.p + SC_TMP0);
+        hipLaunchKernelGGL(k_norm_power<double>, dim3(nb), dim3(BS), 0, s, (const double*)Ax, (const double*)dinv,
+                           (double*)x, n, red.part(PB_MISC));
     else
-        hipLaunchKernelGGL(k_norm_power<float>, dim3(red_blocks(n)), dim3(BS), 0, s, (const float*)Ax,
-                           (const float*)dinv, (float*)x, n, red.partials.p, red.counter.p, red.scal.p + SC_TMP0);
+        hipLaunchKernelGGL(k_norm_power<float>, dim3(nb), dim3(BS), 0, s, (const float*)Ax, (const float*)dinv,
+                           (float*)x, n, red.part(PB_MISC));
+    red.nblk[PB_MISC] = nb;
+    red.nv[PB_MISC] = 1;
     TT_HIP(hipGetLastError());
 }
 
+// x /= ||x|| with ||x||^2 summed from the partials of k_norm_power
 template <typename T>
-__global__ void k_scale_by_scal(T* x, long n, const double* scal, int slot) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) x[i] = (T)((double)x[i] / sqrt(scal[slot]));
+__global__ void k_scale_by_norm(T* x, long n, const double* part, int nblk) {
+    double s[1];
+    sum_partials<1>(part, nblk, s);
+    const double inv = 1.0 / sqrt(s[0]);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = (T)((double)x[i] * inv);
 }
-void launch_scale_by_scal(void* x, long n, const double* scal, int slot, bool fp64, hipStream_t s) {
-    if (fp64) hipLaunchKernelGGL(k_scale_by_scal<double>, dim3(nblocks(n)), dim3(BS), 0, s, (double*)x, n, scal, slot);
-    else hipLaunchKernelGGL(k_scale_by_scal<float>, dim3(nblocks(n)), dim3(BS), 0, s, (float*)x, n, scal, slot);
+void launch_scale_by_norm(void* x, long n, Reducer& red, bool fp64, hipStream_t s) {
+    const int nb = rb_blocks(n);
+    if (fp64)
+        hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(nb), dim3(BS), 0, s, (double*)x, n, red.part(PB_MISC),
+                           red.nblk[PB_MISC]);
+    else
+        hipLaunchKernelGGL(k_scale_by_norm<float>, dim3(nb), dim3(BS), 0, s, (float*)x, n, red.part(PB_MISC),
+                           red.nblk[PB_MISC]);
     TT_HIP(hipGetLastError());
 }
 

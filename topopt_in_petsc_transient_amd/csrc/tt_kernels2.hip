@@ -8,7 +8,7 @@ constexpr int BS2 = 256;
 static inline int nblk(long n, int per = BS2) { return (int)((n + per - 1) / per); }
 static inline int rblk(long n) {
     int b = nblk(n);
-    return b > 2048 ? 2048 : (b < 1 ? 1 : b);
+    return b > 256 ? 256 : (b < 1 ? 1 : b);
 }
 
 // ---- deterministic reduction (same protocol as tt_kernels.hip)
@@ -35,35 +35,20 @@ __device__ __forceinline__ void bsum(double (&v)[NV], double* sh) {
         }
     }
 }
+// block partial sums -> partials[block*NV + k]; summed later in block order by the host
+// (Reducer::host_sum) or by the next kernel
 template <int NV>
-__device__ __forceinline__ void rfinish(double (&v)[NV], double* partials, unsigned* counter, double* out) {
+__device__ __forceinline__ void rfinish(double (&v)[NV], double* partials, unsigned*, double*) {
     __shared__ double sh[16 * NV];
-    __shared__ bool last;
     bsum<NV>(v, sh);
-    const unsigned nb = gridDim.x;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) partials[(size_t)blockIdx.x * NV + k] = v[k];
-        __threadfence();
-        last = (atomicAdd(counter, 1u) == nb - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-            acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    bsum<NV>(acc, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) out[k] = acc[k];
-        *counter = 0u;
-    }
+}
+static inline void set_region(Reducer& red, int region, int nb, int nv) {
+    red.nblk[region] = nb;
+    red.nv[region] = nv;
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
@@ -241,12 +226,13 @@ void launch_sens(const SensArgs& a, Reducer& red, double* fx_out, hipStream_t st
         s.u = a.u; s.s1 = a.v; s.s2 = a.a; s.s3 = a.a;
     }
     for (int k = 0; k < 4; k++) { s.wK[k] = a.wK[k]; s.wM[k] = a.wM[k]; }
+    // fx_out receives one partial objective value per block (nblk(nel) of them)
     const int nb = nblk(a.g.nel);
-    TT_CHECK(nb <= red.maxblocks, -3, "reducer too small for sensitivity kernel");
+    (void)red;
     if (a.objective == 3)
-        hipLaunchKernelGGL(k_sens<true>, dim3(nb), dim3(BS2), 0, st, s, red.partials.p, red.counter.p, fx_out);
+        hipLaunchKernelGGL(k_sens<true>, dim3(nb), dim3(BS2), 0, st, s, fx_out, (unsigned*)nullptr, (double*)nullptr);
     else
-        hipLaunchKernelGGL(k_sens<false>, dim3(nb), dim3(BS2), 0, st, s, red.partials.p, red.counter.p, fx_out);
+        hipLaunchKernelGGL(k_sens<false>, dim3(nb), dim3(BS2), 0, st, s, fx_out, (unsigned*)nullptr, (double*)nullptr);
     TT_HIP(hipGetLastError());
 }
 
@@ -286,8 +272,9 @@ void launch_filter_apply(const Grid& g, const double* kf, const double* u, doubl
     const int nb = nblk(g.nn);
     if (red) {
         TT_CHECK(nb <= red->maxblocks, -3, "reducer too small for filter apply");
-        hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(BS2), 0, s, ga(g), kf, u, y, red->partials.p,
-                           red->counter.p, red->scal.p + dot_slot);
+        hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(BS2), 0, s, ga(g), kf, u, y, red->part(dot_slot),
+                           (unsigned*)nullptr, (double*)nullptr);
+        set_region(*red, dot_slot, nb, 1);
     } else {
         hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(BS2), 0, s, ga(g), kf, u, y, (double*)nullptr,
                            (unsigned*)nullptr, (double*)nullptr);
@@ -387,14 +374,16 @@ __global__ void k_sum(const double* __restrict__ x, long n, int mnd, double* par
         acc[0] += mnd ? 4 * x[i] * (1.0 - x[i]) : x[i];
     rfinish<1>(acc, partials, counter, out);
 }
-void launch_sum(const double* x, long n, Reducer& red, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 0, red.partials.p, red.counter.p,
-                       red.scal.p + slot);
+void launch_sum(const double* x, long n, Reducer& red, int region, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 0, red.part(region), (unsigned*)nullptr,
+                       (double*)nullptr);
+    set_region(red, region, rblk(n), 1);
     TT_HIP(hipGetLastError());
 }
-void launch_mnd(const double* x, long n, Reducer& red, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 1, red.partials.p, red.counter.p,
-                       red.scal.p + slot);
+void launch_mnd(const double* x, long n, Reducer& red, int region, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 1, red.part(region), (unsigned*)nullptr,
+                       (double*)nullptr);
+    set_region(red, region, rblk(n), 1);
     TT_HIP(hipGetLastError());
 }
 
@@ -456,7 +445,8 @@ void launch_mma_gensub(const MMADev& m, const double* x, const double* df, const
                        const double* xmax, int k, double asyminit, double asymdec, double asyminc, Reducer& red,
                        hipStream_t s) {
     hipLaunchKernelGGL(k_mma_gensub, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), x, df, dg, xmin, xmax, k, asyminit,
-                       asymdec, asyminc, red.partials.p, red.counter.p, red.scal.p + SC_TMP0);
+                       asymdec, asyminc, red.part(PB_MISC), (unsigned*)nullptr, (double*)nullptr);
+    set_region(red, PB_MISC, rblk(m.n), 1);
     TT_HIP(hipGetLastError());
 }
 
@@ -481,8 +471,9 @@ __global__ void k_mma_xgh(MD m, double lam, double* __restrict__ x, double* part
     rfinish<2>(acc, partials, counter, out);
 }
 void launch_mma_xyz_grad_hess(const MMADev& m, double lam, double* x, Reducer& red, hipStream_t s) {
-    hipLaunchKernelGGL(k_mma_xgh, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), lam, x, red.partials.p, red.counter.p,
-                       red.scal.p + SC_TMP0);
+    hipLaunchKernelGGL(k_mma_xgh, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), lam, x, red.part(PB_MISC),
+                       (unsigned*)nullptr, (double*)nullptr);
+    set_region(red, PB_MISC, rblk(m.n), 2);
     TT_HIP(hipGetLastError());
 }
 // XYZofLAMBDA + DualResidual sum (MMA.cc:893-934)
@@ -500,8 +491,9 @@ __global__ void k_mma_xres(MD m, double lam, double* __restrict__ x, double* par
     rfinish<1>(acc, partials, counter, out);
 }
 void launch_mma_xyz_res(const MMADev& m, double lam, double* x, Reducer& red, hipStream_t s) {
-    hipLaunchKernelGGL(k_mma_xres, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), lam, x, red.partials.p, red.counter.p,
-                       red.scal.p + SC_TMP0);
+    hipLaunchKernelGGL(k_mma_xres, dim3(rblk(m.n)), dim3(BS2), 0, s, md(m), lam, x, red.part(PB_MISC),
+                       (unsigned*)nullptr, (double*)nullptr);
+    set_region(red, PB_MISC, rblk(m.n), 1);
     TT_HIP(hipGetLastError());
 }
 
@@ -564,8 +556,9 @@ __global__ void k_mma_kkt(const double* __restrict__ x, const double* __restrict
 void launch_mma_kkt(const double* x, const double* df, const double* dg, const double* xmin, const double* xmax,
                     long n, double lam, Reducer& red, hipStream_t s) {
     TT_HIP(hipMemsetAsync(red.scal.p + SC_TMP2, 0, sizeof(double), s));
-    hipLaunchKernelGGL(k_mma_kkt, dim3(rblk(n)), dim3(BS2), 0, s, x, df, dg, xmin, xmax, n, lam, red.partials.p,
-                       red.counter.p, red.scal.p + SC_TMP1, red.scal.p + SC_TMP2);
+    hipLaunchKernelGGL(k_mma_kkt, dim3(rblk(n)), dim3(BS2), 0, s, x, df, dg, xmin, xmax, n, lam, red.part(PB_MISC),
+                       (unsigned*)nullptr, (double*)nullptr, red.scal.p + SC_TMP2);
+    set_region(red, PB_MISC, rblk(n), 1);
     TT_HIP(hipGetLastError());
 }
 

@@ -71,6 +71,15 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
         TT_HIP(hipMemcpy(lev->M864.p, m8, sizeof(m8), hipMemcpyHostToDevice));
         TT_HIP(hipMemcpy(lev->KE32.p, ke32, sizeof(ke32), hipMemcpyHostToDevice));
         TT_HIP(hipMemcpy(lev->M832.p, m832, sizeof(m832), hipMemcpyHostToDevice));
+        {
+            double af64[18 * 64];
+            float af32[18 * 64];
+            mfma_fragments(ke, m8, af64, af32);
+            lev->AF64.alloc(18 * 64);
+            lev->AF32.alloc(18 * 64);
+            TT_HIP(hipMemcpy(lev->AF64.p, af64, sizeof(af64), hipMemcpyHostToDevice));
+            TT_HIP(hipMemcpy(lev->AF32.p, af32, sizeof(af32), hipMemcpyHostToDevice));
+        }
         if (l > 0) {
             lev->kf64.alloc(g.nel);
             lev->mf64.alloc(g.nel);
@@ -91,6 +100,8 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
         lev->op.M864 = lev->M864.p;
         lev->op.KE32 = lev->KE32.p;
         lev->op.M832 = lev->M832.p;
+        lev->op.AF64 = lev->AF64.p;
+        lev->op.AF32 = lev->AF32.p;
         L.push_back(std::move(lev));
     }
     const Grid& gc = L.back()->g;
@@ -165,9 +176,9 @@ double Multigrid::power_lmax(int l, int iters) {
     for (int it = 0; it < iters; it++) {
         launch_apply_bc(lv.op, cK, cM, lv.x.p, lv.r.p, lv.fp64, s);
         launch_norm_power(lv.r.p, lv.dinv.p, lv.x.p, nd, ctx->red, lv.fp64, s);
-        launch_scale_by_scal(lv.x.p, nd, ctx->red.scal.p, SC_TMP0, lv.fp64, s);
+        launch_scale_by_norm(lv.x.p, nd, ctx->red, lv.fp64, s);
     }
-    return std::sqrt(ctx->read_scal(SC_TMP0));
+    return std::sqrt(ctx->reduce1(PB_MISC));
 }
 
 // Cholesky-based inverse of the dense SPD coarsest operator (host, FP64, row-major, contiguous inner loops)
@@ -288,9 +299,9 @@ void Multigrid::setup_operator(double cK_, double cM_) {
         for (int it = 0; it < 20; it++) {
             launch_apply_bc(op, cK, cM, cx.p, cd.p, true, s);
             launch_norm_power(cd.p, cdinv.p, cx.p, nc, ctx->red, true, s);
-            launch_scale_by_scal(cx.p, nc, ctx->red.scal.p, SC_TMP0, true, s);
+            launch_scale_by_norm(cx.p, nc, ctx->red, true, s);
         }
-        clmax = 1.05 * std::sqrt(ctx->read_scal(SC_TMP0));
+        clmax = 1.05 * std::sqrt(ctx->reduce1(PB_MISC));
         clmin = clmax / 30.0;
     }
 }
@@ -410,6 +421,12 @@ double Context::read_scal(int slot) {
     return red.host[slot];
 }
 
+double Context::reduce1(int region) {
+    double v[4];
+    red.host_sum(region, stream, v);
+    return v[0];
+}
+
 OpLevel Context::fine_op(bool fp64) const {
     OpLevel o = op_fine64;
     if (!fp64) {
@@ -425,11 +442,10 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
     const OpLevel& op = op_fine64;
     const double rtol = prm.rtol;
     launch_residual(op, cK, cM, b, x, kr.p, true, stream);
-    launch_dot(b, b, nd, red, SC_TMP1, stream);
-    launch_dot(kr.p, kr.p, nd, red, SC_RR, stream);
-    TT_HIP(hipMemcpyAsync(red.host, red.scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, stream));
-    sync();
-    const double bn = std::sqrt(red.host[SC_TMP1]);
+    launch_dot(b, b, nd, red, PB_MISC, stream);
+    launch_dot(kr.p, kr.p, nd, red, PB_MISC2, stream);
+    const double bn = std::sqrt(reduce1(PB_MISC));
+    const double rr0 = reduce1(PB_MISC2);
     SolveResult res;
     stats.solves++;
     if (!std::isfinite(bn)) throw Error{TT_ERR_NONFINITE, "non-finite right-hand side"};
@@ -439,7 +455,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
         stats.last_relres = 0.0;
         return res;
     }
-    double rel = std::sqrt(red.host[SC_RR]) / bn;
+    double rel = std::sqrt(rr0) / bn;
     bool first = true, conv = rel <= rtol;
     double* pold = kp0.p;
     double* pnew = kp1.p;
@@ -453,13 +469,13 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
         std::swap(pold, pnew);
         first = false;
         it++;
-        rel = std::sqrt(read_scal(SC_RR)) / bn;
+        rel = std::sqrt(reduce1(PB_RR)) / bn;
         if (!std::isfinite(rel)) throw Error{TT_ERR_NONFINITE, "non-finite residual in flexible CG"};
         if (rel <= rtol) {
             // confirm with the true residual; restart the recursion if it drifted
             launch_residual(op, cK, cM, b, x, kr.p, true, stream);
-            launch_dot(kr.p, kr.p, nd, red, SC_RR, stream);
-            rel = std::sqrt(read_scal(SC_RR)) / bn;
+            launch_dot(kr.p, kr.p, nd, red, PB_MISC, stream);
+            rel = std::sqrt(reduce1(PB_MISC)) / bn;
             if (rel <= rtol) conv = true;
             else first = true;
         }

@@ -210,6 +210,7 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     const double al = prm.dalpha, be = prm.dbeta;
     const int obj = prm.objective;
     const double coef[8] = {a[0], a[1], a[2], a[3], a[4], a[5], al, be};
+    const long nbs = (g.nel + 255) / 256;  // objective partials per time step (one per sensitivity block)
     launch_fill(df.p, 0.0, g.nel, true, stream);
     launch_fill(hdu.p, 0.0, nd, true, stream);
     launch_fill(hddu.p, 0.0, nd, true, stream);
@@ -262,7 +263,7 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
         sa.objective = obj;
         sa.mode = mode;
         sa.df = df.p;
-        launch_sens(sa, red, fxs.p + t, stream);
+        launch_sens(sa, red, fxs.p + (size_t)t * nbs, stream);
     };
 
     // one backward step at t: needs state t and state t-1
@@ -309,10 +310,13 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     solve_M(rhs.p, nu0.p);
     sens(1, nu0.p, s0, nullptr, 0);
     // objective: sum over t = 1..T, then t = 0 (:1778-1802)
-    std::vector<double> f(T + 1);
-    TT_HIP(hipMemcpyAsync(f.data(), fxs.p, sizeof(double) * (T + 1), hipMemcpyDeviceToHost, stream));
+    std::vector<double> fp((size_t)(T + 1) * nbs);
+    TT_HIP(hipMemcpyAsync(fp.data(), fxs.p, sizeof(double) * fp.size(), hipMemcpyDeviceToHost, stream));
     if (dfdx_dev) launch_copy(df.p, dfdx_dev, g.nel, true, true, stream);
     sync();
+    std::vector<double> f(T + 1, 0.0);
+    for (int t = 0; t <= T; t++)
+        for (long b = 0; b < nbs; b++) f[t] += fp[(size_t)t * nbs + b];
     double fx = 0.0;
     for (int t = 1; t <= T; t++) fx += f[t];
     fx += f[0];
