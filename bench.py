@@ -33,14 +33,18 @@ METRIC = "Newmark steps/s + adjoint-gradient time, 128x64x64 hex8, mixed vs FP64
 PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 PEAK_FP64 = 78.6e12        # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_FP32 = 157.3e12       # MI355X dense FP32 matrix, flop/s
+# HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
+# scaled by the counters' reading for a copy kernel of known size); None until measured.
+TRAFFIC_A32 = None
 NEL = (128, 64, 64)
 DT = 0.005
+NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
 
 
 def params(capi, precision, nsteps, slots=0, nlvls=None):
     return capi.default_params(
         nelx=NEL[0], nely=NEL[1], nelz=NEL[2], xc=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0), nsteps=nsteps, t1=DT * nsteps,
-        pulse_steps=20, precision=precision, rtol=1e-8, maxit=200, nlvls=nlvls or int(os.environ.get("TT_NLVLS", 5)),
+        pulse_steps=20, precision=precision, rtol=1e-8, maxit=200, nlvls=nlvls or NLVLS,
         cheb_degree=2, checkpoint_slots=slots, rmin=0.08, filter_rtol=1e-10)
 
 
@@ -89,9 +93,15 @@ def op_microbench(capi, torch, nrep):
             "fp64_ms": ms64, "fp32_ms": ms32, "fp64_applies_per_s": 1e3 / ms64, "fp32_applies_per_s": 1e3 / ms32,
             "fp32_rel_err_vs_fp64": err, "flops": fl, "fp64_tflops": fl / (ms64 * 1e-3) / 1e12,
             "fp32_tflops": fl / (ms32 * 1e-3) / 1e12, "fp64_GBs": b64 / (ms64 * 1e-3) / 1e9,
-            "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9,
+            "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9, "nel": nel,
         }
     ctx.close()
+    # calibration for the HBM traffic counters: a plain device copy of 64 MiB (64 MiB read + 64 MiB written)
+    a = torch.empty(1 << 24, dtype=torch.float32, device=dev).normal_()
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
     return out
 
 
@@ -152,7 +162,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--ops-only", action="store_true", help="only the operator microbenchmark (for profiling)")
+    ap.add_argument("--nlvls", type=int, default=4, help="multigrid levels (4: coarsest grid 16x8x8 elements)")
     args = ap.parse_args()
+    global NLVLS
+    NLVLS = args.nlvls
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU domain decomposition is a design note only).", file=sys.stderr)
@@ -195,13 +208,16 @@ def main():
         log("CPU oracle sample")
         cpu = cpu_baseline(args.cpu_steps)
 
-    # roofline of the dominant kernel: the FP32 Newmark-operator apply of the V-cycle smoother
+    # roofline of the dominant kernel: the FP32 Newmark-operator apply of the V-cycle smoother.
+    # Algorithmic work per launch: the stacked [K; M] 48x24 element contraction, 2*1152 flops per element.
     a32 = ops["A"]
-    ach = a32["flops"] / (a32["fp32_ms"] * 1e-3) / 1e12
+    fl = 2 * 1152 * a32["nel"]
+    ach = fl / (a32["fp32_ms"] * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-            "frac": ach / (PEAK_FP32 / 1e12), "traffic": None,
-            "kernel": "k_apply<float,float,...> (A = K + a2 C + a5 M, FP32, 128x64x64)",
-            "kernel_ms": a32["fp32_ms"]}
+            "frac": ach / (PEAK_FP32 / 1e12), "traffic": TRAFFIC_A32,
+            "kernel": "k_apply_mfma<float,float,4,...,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
+                      "128x64x64)",
+            "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl}
     value = args.steps / mixed["seconds"]
     rec = {
         "metric": METRIC, "value": value, "unit": "Newmark steps/s", "n_gpus": 1, "steps": args.steps,
@@ -209,7 +225,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64 (mixed: FP32/BF16 multigrid, FP64 outer CG)",
         "data": "synthetic (seeded filtered-noise density, half-sine line load)",
         "config": {"workload": "128x64x64 hex8 cantilever, Newmark dt=0.005, 4-level... see nlvls",
-                   "nlvls": int(os.environ.get("TT_NLVLS", 5)), "rtol": 1e-8, "cheb_degree": 2,
+                   "nlvls": NLVLS, "rtol": 1e-8, "cheb_degree": 2,
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": args.steps / fp64["seconds"], "mixed_vs_fp64_speedup": fp64["seconds"] / mixed["seconds"],
         "u_final_rel_err_mixed_vs_fp64": u_err, "forward_mixed": mixed, "forward_fp64": fp64,

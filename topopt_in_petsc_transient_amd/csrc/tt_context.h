@@ -29,7 +29,7 @@ struct MGLevel {
 };
 
 // Geometric multigrid preconditioner (V-cycle, Chebyshev smoothing on D^-1 A,
-// rediscretised coarse operators, dense FP64 coarsest solve).
+// rediscretised coarse operators, dense FP64 coarsest solve through an explicit inverse).
 struct Multigrid {
     Context* ctx = nullptr;
     std::vector<std::unique_ptr<MGLevel>> L;
@@ -41,8 +41,15 @@ struct Multigrid {
     int nc = 0;
     bool dense = true;
     DBuf<double> Ainv, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest vectors
+    DBuf<int> dinfo;
+    void* blas = nullptr;  // rocblas_handle used by the rocSOLVER factorisation of the coarsest operator
     int coarse_deg = 12;
     double clmin = 0.0, clmax = 1.0;
+
+    Multigrid() = default;
+    Multigrid(const Multigrid&) = delete;
+    Multigrid& operator=(const Multigrid&) = delete;
+    ~Multigrid();
 
     void build(Context* c, int nlvls, bool mixed_prec, int deg);
     void set_coefficients(const double* kf_fine, const double* mf_fine);

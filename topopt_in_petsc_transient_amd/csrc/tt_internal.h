@@ -148,6 +148,8 @@ struct OpLevel {
 };
 
 // MFMA A fragments of the interleaved [KE; M8 (x) I3] matrix, lane order (see tt_kernels.hip)
+// Column-major lower triangle of an n x n matrix copied into its upper triangle.
+void launch_mirror_lower(double* A, int n, hipStream_t s);
 void mfma_fragments(const double* KE, const double* M8, double* af64, float* af32);
 
 // y = N.(cK*K + cM*M)(N.u) on free dofs, identity on clamped dofs.  With red: partials of u.y in

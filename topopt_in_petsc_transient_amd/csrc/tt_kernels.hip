@@ -65,29 +65,35 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* part) {
     }
 }
 
-// Sum of partials written by an earlier kernel, in block order; every thread gets the totals.
-// One wave reduces: lane l sums blocks l, l+64, ... then a fixed butterfly.
+// Sum of partials written by an earlier kernel; every thread of the block gets the totals.
+// Thread t sums blocks t, t + blockDim, ... (a fixed order for a given block size), then the
+// waves are combined by a fixed butterfly and a fixed wave order, so every block of a launch
+// obtains bit-identical totals.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nblk, double (&out)[NV]) {
-    __shared__ double res[NV];
-    if (threadIdx.x < 64) {
-        double acc[NV];
+    __shared__ double sh[16 * NV];
+    double acc[NV];
 #pragma unroll
-        for (int k = 0; k < NV; k++) acc[k] = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += 64)
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
 #pragma unroll
-            for (int k = 0; k < NV; k++) acc[k] += part[(size_t)b * NV + k];
+        for (int k = 0; k < NV; k++) acc[k] += part[(size_t)b * NV + k];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x + 63) >> 6;
 #pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double x = acc[k];
+    for (int k = 0; k < NV; k++) {
+        double x = acc[k];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-            if (threadIdx.x == 0) res[k] = x;
-        }
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) sh[wid * NV + k] = x;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NV; k++) out[k] = res[k];
+    for (int k = 0; k < NV; k++) {
+        double x = 0.0;
+        for (int w = 0; w < nw; w++) x += sh[w * NV + k];
+        out[k] = x;
+    }
+    __syncthreads();  // sh may be reused by a later call in the same kernel
 }
 
 void Reducer::init(int maxb) {
@@ -181,26 +187,15 @@ struct StageOne {  // w = u
     __device__ __forceinline__ T k(int i) const { return u[i]; }
     __device__ __forceinline__ T m(int i) const { return u[i]; }
 };
-// Flexible-CG direction p = z + beta p_old with beta = z.(r - r_prev) / gamma_old, the two
-// dots summed from the partials of the preceding k_fcg_dots.
+// Flexible-CG direction p = z + beta p_old, beta = z.(r - r_prev) / gamma_old computed by k_fcg_beta.
 template <typename T>
 struct StageFcg {
     const T* z;
     const T* pold;
-    const double* dots_part;
-    int dots_nblk;
     const double* scal;
     int first;
     T beta;
-    __device__ __forceinline__ void prologue() {
-        if (first) {
-            beta = 0;
-            return;
-        }
-        double d[2];
-        sum_partials<2>(dots_part, dots_nblk, d);
-        beta = (T)((d[0] - d[1]) / scal[SC_GAMMA_OLD]);
-    }
+    __device__ __forceinline__ void prologue() { beta = first ? (T)0 : (T)scal[SC_BETA]; }
     __device__ __forceinline__ T k(int i) const { return first ? z[i] : z[i] + beta * pold[i]; }
     __device__ __forceinline__ T m(int i) const { return k(i); }
 };
@@ -294,16 +289,28 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
 
 #include "tt_apply_mfma.inc"
 
-// ---- epilogues.  All are called by every thread (inside or not) so that reducing epilogues can sync.
+// ---- epilogues.  put() handles one owned node (inside the grid) and may add to the thread's
+// reduction share; done() is called once by every thread of the block at the end (reducing
+// epilogues write their block partial there).  operator() is the one-node-per-thread form.
+#define TT_EPI_CALL                                                                                      \
+    template <class S, typename V>                                                                     \
+    __device__ void operator()(bool in, int n, uint8_t cl, const V (&yK)[3], const V (&yM)[3],          \
+                               const S& st) const {                                                     \
+        double acc = 0.0;                                                                              \
+        if (in) put(n, cl, yK, yM, st, acc);                                                           \
+        done(acc);                                                                                     \
+    }
+
 template <typename T>
 struct EpiBC {  // y = A u on free dofs, u on clamped
     T* y;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
-        if (!in) return;
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
     }
+    __device__ void done(double) const {}
+    TT_EPI_CALL
 };
 
 template <typename T>
@@ -311,19 +318,20 @@ struct EpiBCDot {  // y = A u (BC form) and partials of u.y
     T* y;
     double* part;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
-        double acc[1] = {0.0};
-        if (in) {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double& acc) const {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const T u = st.k(3 * n + c);
-                const T v = ((cl >> c) & 1) ? u : yK[c] + yM[c];
-                y[3 * n + c] = v;
-                acc[0] += (double)u * (double)v;
-            }
+        for (int c = 0; c < 3; c++) {
+            const T u = st.k(3 * n + c);
+            const T v = ((cl >> c) & 1) ? u : yK[c] + yM[c];
+            y[3 * n + c] = v;
+            acc += (double)u * (double)v;
         }
-        write_partials<1>(acc, part);
     }
+    __device__ void done(double acc) const {
+        double a[1] = {acc};
+        write_partials<1>(a, part);
+    }
+    TT_EPI_CALL
 };
 
 template <typename T>
@@ -331,14 +339,15 @@ struct EpiResid {  // r = b - A x
     const T* b;
     T* r;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
-        if (!in) return;
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const T ax = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
             r[3 * n + c] = b[3 * n + c] - ax;
         }
     }
+    __device__ void done(double) const {}
+    TT_EPI_CALL
 };
 
 template <typename T>
@@ -350,8 +359,7 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
     T* xn;
     T* dn;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st) const {
-        if (!in) return;
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -364,15 +372,15 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
             if (dn) dn[i] = dd;
         }
     }
+    __device__ void done(double) const {}
+    TT_EPI_CALL
 };
 
 struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
     double* yk;
     double* ym;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const double (&yK)[3], const double (&yM)[3],
-                               const S&) const {
-        if (!in) return;
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const bool clamped = (cl >> c) & 1;
@@ -380,17 +388,19 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
             ym[3 * n + c] = clamped ? 0.0 : yM[c];
         }
     }
+    __device__ void done(double) const {}
+    TT_EPI_CALL
 };
 
 struct EpiRaw {  // y = yK + yM, no BC
     double* y;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t, const double (&yK)[3], const double (&yM)[3],
-                               const S&) const {
-        if (!in) return;
+    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, double&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = yK[c] + yM[c];
     }
+    __device__ void done(double) const {}
+    TT_EPI_CALL
 };
 
 struct EpiRHS {  // R = N.(fs F + yK + yM)
@@ -398,9 +408,7 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     double fs;
     double* R;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const double (&yK)[3], const double (&yM)[3],
-                               const S&) const {
-        if (!in) return;
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -410,6 +418,8 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
             R[i] = ((cl >> c) & 1) ? 0.0 : r;
         }
     }
+    __device__ void done(double) const {}
+    TT_EPI_CALL
 };
 
 struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
@@ -417,23 +427,25 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
     double* q;
     double* part;
     template <class S>
-    __device__ void operator()(bool in, int n,uint8_t cl, const double (&yK)[3], const double (&yM)[3],
-                               const S& st) const {
-        double acc[1] = {0.0};
-        if (in) {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st,
+                        double& acc) const {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const int i = 3 * n + c;
-                const double p = st.k(i);
-                const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
-                pnew[i] = p;
-                q[i] = v;
-                acc[0] += p * v;
-            }
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            const double p = st.k(i);
+            const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
+            pnew[i] = p;
+            q[i] = v;
+            acc += p * v;
         }
-        write_partials<1>(acc, part);
     }
+    __device__ void done(double acc) const {
+        double a[1] = {acc};
+        write_partials<1>(a, part);
+    }
+    TT_EPI_CALL
 };
+#undef TT_EPI_CALL
 
 static inline dim3 apply_grid(const Grid& g) {
     return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + TZ - 1) / TZ);
@@ -449,6 +461,39 @@ static int apply_impl() {
     return v;
 }
 
+static int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        TT_HIP(hipGetDevice(&dev));
+        TT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+// Launch of k_apply_tiles: one block per tile, rounded up to a multiple of 8 for the XCD-aware
+// tile order; returns the block count (= number of reduction partials).
+template <typename T, typename CT, int TZ_, bool MASKIN, class Stage, class Epi>
+static int launch_tiles_ct(const OpLevel& L, const T* AF, int tx, int ty, int tz, T cK, T cM, const Stage& st,
+                           const Epi& epi, hipStream_t s) {
+    const int ntiles = tx * ty * tz;
+    const int nb = 8 * ((ntiles + 7) / 8);
+    hipLaunchKernelGGL((k_apply_tiles<T, CT, TZ_, MASKIN, Stage, Epi>), dim3(nb), dim3(PTile<TZ_>::NTH), 0, s,
+                       gargs(L.g), tx, ty, ntiles, AF, (const CT*)L.kf, (const CT*)L.mf, cK, cM, L.mask, st, epi);
+    TT_HIP(hipGetLastError());
+    return nb;
+}
+template <typename T, int TZ_, bool MASKIN, class Stage, class Epi>
+static int launch_tiles(const OpLevel& L, const T* AF, int tx, int ty, int tz, T cK, T cM, const Stage& st,
+                        const Epi& epi, hipStream_t s) {
+    switch (L.coef_kind) {
+    case COEF_F64: return launch_tiles_ct<T, double, TZ_, MASKIN>(L, AF, tx, ty, tz, cK, cM, st, epi, s);
+    case COEF_F32: return launch_tiles_ct<T, float, TZ_, MASKIN>(L, AF, tx, ty, tz, cK, cM, st, epi, s);
+    default: return launch_tiles_ct<T, bf16_t, TZ_, MASKIN>(L, AF, tx, ty, tz, cK, cM, st, epi, s);
+    }
+}
+
 // dispatch on coefficient kind and precision
 template <typename T, bool TWO, bool MASKIN, class Stage, class Epi>
 static int run_apply(const OpLevel& L, double cK, double cM, const Stage& st, const Epi& epi, hipStream_t s,
@@ -457,32 +502,12 @@ static int run_apply(const OpLevel& L, double cK, double cM, const Stage& st, co
     const T* M8 = std::is_same<T, double>::value ? (const T*)L.M864 : (const T*)L.M832;
     GridArgs ga = gargs(L.g);
     if (!TWO && allow_mfma && apply_impl() == 1) {
-        // FP32: 8x8x4 node tiles (50 KB LDS); FP64: 8x8x2 tiles keep the block under 64 KB of LDS
-        constexpr int TZm = std::is_same<T, double>::value ? 2 : 4;
-        typedef MTile<TZm> Tl;
         const T* AF = std::is_same<T, double>::value ? (const T*)L.AF64 : (const T*)L.AF32;
-        const int tx = (L.g.nx + Tl::X - 1) / Tl::X, ty = (L.g.ny + Tl::Y - 1) / Tl::Y;
-        const int ntiles = tx * ty * ((L.g.nz + Tl::Z - 1) / Tl::Z);
-        dim3 grid(8 * ((ntiles + 7) / 8));  // 1-D, XCD-aware tile order inside the kernel
-        switch (L.coef_kind) {
-        case COEF_F64:
-            hipLaunchKernelGGL((k_apply_mfma<T, double, TZm, MASKIN, Stage, Epi>), grid, dim3(Tl::NT), 0, s, ga, tx,
-                               ty, ntiles, AF, (const double*)L.kf, (const double*)L.mf, (T)cK, (T)cM, L.mask, st,
-                               epi);
-            break;
-        case COEF_F32:
-            hipLaunchKernelGGL((k_apply_mfma<T, float, TZm, MASKIN, Stage, Epi>), grid, dim3(Tl::NT), 0, s, ga, tx,
-                               ty, ntiles, AF, (const float*)L.kf, (const float*)L.mf, (T)cK, (T)cM, L.mask, st,
-                               epi);
-            break;
-        default:
-            hipLaunchKernelGGL((k_apply_mfma<T, bf16_t, TZm, MASKIN, Stage, Epi>), grid, dim3(Tl::NT), 0, s, ga, tx,
-                               ty, ntiles, AF, (const bf16_t*)L.kf, (const bf16_t*)L.mf, (T)cK, (T)cM, L.mask, st,
-                               epi);
-            break;
-        }
-        TT_HIP(hipGetLastError());
-        return (int)grid.x;
+        // 8x8x4 tiles on grids with enough of them to fill the chip, 8x8x2 on the coarse levels
+        const int tx = (L.g.nx + 7) / 8, ty = (L.g.ny + 7) / 8;
+        const bool deep = (long)tx * ty * ((L.g.nz + 3) / 4) >= 4L * device_cus();
+        if (deep) return launch_tiles<T, 4, MASKIN>(L, AF, tx, ty, (L.g.nz + 3) / 4, (T)cK, (T)cM, st, epi, s);
+        return launch_tiles<T, 2, MASKIN>(L, AF, tx, ty, (L.g.nz + 1) / 2, (T)cK, (T)cM, st, epi, s);
     }
     dim3 grid = apply_grid(L.g);
     switch (L.coef_kind) {
@@ -575,10 +600,14 @@ void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, cons
     run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R}, s);
 }
 
+__global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first);
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
                       double* q, Reducer& red, bool first, hipStream_t s) {
     TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
-    StageFcg<double> st{z, p_old, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p, first ? 1 : 0, 0.0};
+    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BS), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
+                       first ? 1 : 0);
+    TT_HIP(hipGetLastError());
+    StageFcg<double> st{z, p_old, red.scal.p, first ? 1 : 0, 0.0};
     EpiFcg e{p_new, q, red.part(PB_PQ)};
     red.nblk[PB_PQ] = run_apply<double, false, true>(L, cK, cM, st, e, s);
     red.nv[PB_PQ] = 1;
@@ -848,6 +877,26 @@ __global__ void k_dense_gemv(const double* __restrict__ A, const double* __restr
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) x[row] = s;
 }
+__global__ void k_mirror_lower(double* A, int n) {
+    __shared__ double t[32][33];
+    const int bi = blockIdx.x, bj = blockIdx.y;  // tile (bi, bj) of the lower part, bi >= bj
+    if (bi < bj) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    for (int r = ty; r < 32; r += 8) {
+        const int i = bi * 32 + tx, j = bj * 32 + r;  // element (row i, col j) at A[i + j n]
+        t[r][tx] = (i < n && j < n) ? A[(size_t)j * n + i] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = bj * 32 + tx, i = bi * 32 + r;  // write (row j, col i) = (row i, col j)
+        if (i < n && j < n && i > j) A[(size_t)i * n + j] = t[tx][r];
+    }
+}
+void launch_mirror_lower(double* A, int n, hipStream_t s) {
+    const int nt = (n + 31) / 32;
+    hipLaunchKernelGGL(k_mirror_lower, dim3(nt, nt), dim3(256), 0, s, A, n);
+    TT_HIP(hipGetLastError());
+}
 void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4), dim3(256), 0, s, Ainv, b, x, n);
     TT_HIP(hipGetLastError());
@@ -856,7 +905,7 @@ void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hi
 // ------------------------------------------------------------------ reductions / Krylov
 // Reductions over long vectors use at most RB blocks (grid-stride), so that the
 // consumers of their partials (next kernel or host) sum few values.
-constexpr int RB = 256;
+constexpr int RB = 1024;
 static inline int rb_blocks(long n) {
     int b = nblocks(n);
     return b > RB ? RB : (b < 1 ? 1 : b);
@@ -886,6 +935,15 @@ __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restric
     }
     write_partials<2>(acc, part);
 }
+// gamma = z.r and beta = (z.r - z.r_prev) / gamma_old (0 on a (re)start) from the partials of k_fcg_dots
+__global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first) {
+    double d[2];
+    sum_partials<2>(dots_part, dots_nblk, d);
+    if (threadIdx.x == 0) {
+        scal[SC_GAMMA] = d[0];
+        scal[SC_BETA] = first ? 0.0 : (d[0] - d[1]) / scal[SC_GAMMA_OLD];
+    }
+}
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s) {
     const int nb = rb_blocks(n);
@@ -895,20 +953,22 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
     TT_HIP(hipGetLastError());
 }
 
-// x += alpha p; r_prev = r; r -= alpha q with alpha = gamma / (p.q) summed from the partials of
-// k_fcg_dots and of the FCG apply; partials of r.r.  Block 0 keeps gamma for the next beta.
+// alpha = gamma / (p.q) from the partials of the FCG apply (one block); gamma becomes gamma_old.
+__global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, double* scal) {
+    double pq[1];
+    sum_partials<1>(pq_part, pq_nblk, pq);
+    if (threadIdx.x == 0) {
+        const double gam = scal[SC_GAMMA];
+        scal[SC_ALPHA] = gam / pq[0];
+        scal[SC_GAMMA_OLD] = gam;
+    }
+}
+
+// x += alpha p; r_prev = r; r -= alpha q; partials of r.r.
 __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ p, double* __restrict__ r,
                              double* __restrict__ rp, const double* __restrict__ q, long n,
-                             const double* __restrict__ dots_part, int dots_nblk, const double* __restrict__ pq_part,
-                             int pq_nblk, double* part, double* scal) {
-    double d[2], pq[1];
-    sum_partials<2>(dots_part, dots_nblk, d);
-    sum_partials<1>(pq_part, pq_nblk, pq);
-    const double al = d[0] / pq[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[SC_GAMMA_OLD] = d[0];
-        scal[SC_ALPHA] = al;
-    }
+                             const double* __restrict__ scal, double* part) {
+    const double al = scal[SC_ALPHA];
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         x[i] = fma(al, p[i], x[i]);
@@ -922,9 +982,10 @@ __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ 
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(BS), 0, s, red.part(PB_PQ), red.nblk[PB_PQ], red.scal.p);
+    TT_HIP(hipGetLastError());
     const int nb = rb_blocks(n);
-    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.part(PB_DOTS),
-                       red.nblk[PB_DOTS], red.part(PB_PQ), red.nblk[PB_PQ], red.part(PB_RR), red.scal.p);
+    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.scal.p, red.part(PB_RR));
     red.nblk[PB_RR] = nb;
     red.nv[PB_RR] = 1;
     TT_HIP(hipGetLastError());

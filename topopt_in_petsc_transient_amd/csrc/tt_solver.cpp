@@ -17,6 +17,9 @@
 #include <cmath>
 #include <cstring>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "tt_context.h"
 
 namespace tt {
@@ -106,11 +109,16 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
     }
     const Grid& gc = L.back()->g;
     nc = (int)gc.nd;
-    dense = nc <= 2400;
+    dense = (long)nc * nc * 8 <= (320l << 20);  // explicit FP64 inverse up to 320 MB
     cb.alloc(nc);
     cx.alloc(nc);
     if (dense) {
         Ainv.alloc((size_t)nc * nc);
+        dinfo.alloc(1);
+        rocblas_handle h = nullptr;
+        TT_CHECK(rocblas_create_handle(&h) == rocblas_status_success, TT_ERR_HIP, "rocblas_create_handle failed");
+        blas = h;
+        TT_CHECK(rocblas_set_stream(h, ctx->stream) == rocblas_status_success, TT_ERR_HIP, "rocblas_set_stream failed");
     } else {
         cdinv.alloc(nc);
         cx2.alloc(nc);
@@ -181,46 +189,8 @@ double Multigrid::power_lmax(int l, int iters) {
     return std::sqrt(ctx->reduce1(PB_MISC));
 }
 
-// Cholesky-based inverse of the dense SPD coarsest operator (host, FP64, row-major, contiguous inner loops)
-static void spd_inverse(std::vector<double>& A, int n) {
-    for (int j = 0; j < n; j++) {  // A = L L^T in the lower triangle
-        double* Aj = &A[(size_t)j * n];
-        double d = Aj[j];
-        for (int k = 0; k < j; k++) d -= Aj[k] * Aj[k];
-        TT_CHECK(d > 0.0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
-        d = std::sqrt(d);
-        Aj[j] = d;
-        for (int i = j + 1; i < n; i++) {
-            double* Ai = &A[(size_t)i * n];
-            double s = Ai[j];
-            for (int k = 0; k < j; k++) s -= Ai[k] * Aj[k];
-            Ai[j] = s / d;
-        }
-    }
-    std::vector<double> W((size_t)n * n, 0.0);  // W = L^-1, row by row
-    for (int i = 0; i < n; i++) {
-        const double* Ai = &A[(size_t)i * n];
-        double* Wi = &W[(size_t)i * n];
-        Wi[i] = 1.0;
-        for (int m = 0; m < i; m++) {
-            const double l = Ai[m];
-            const double* Wm = &W[(size_t)m * n];
-            for (int k = 0; k <= m; k++) Wi[k] -= l * Wm[k];
-        }
-        const double inv = 1.0 / Ai[i];
-        for (int k = 0; k <= i; k++) Wi[k] *= inv;
-    }
-    std::fill(A.begin(), A.end(), 0.0);  // A^-1 = W^T W, accumulated by rows of W
-    for (int k = 0; k < n; k++) {
-        const double* Wk = &W[(size_t)k * n];
-        for (int i = 0; i <= k; i++) {
-            const double w = Wk[i];
-            double* Ai = &A[(size_t)i * n];
-            for (int j = 0; j <= i; j++) Ai[j] += w * Wk[j];
-        }
-    }
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < i; j++) A[(size_t)j * n + i] = A[(size_t)i * n + j];
+Multigrid::~Multigrid() {
+    if (blas) (void)rocblas_destroy_handle((rocblas_handle)blas);
 }
 
 void Multigrid::assemble_coarse_inverse() {
@@ -265,9 +235,24 @@ void Multigrid::assemble_coarse_inverse() {
                 }
                 A[(size_t)d * n + d] = 1.0;
             }
-    spd_inverse(A, n);
-    TT_HIP(hipMemcpyAsync(Ainv.p, A.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, ctx->stream));
-    TT_HIP(hipStreamSynchronize(ctx->stream));
+    // A^-1 on the device: Cholesky (potrf) then the inverse from the factor (potri), lower
+    // triangle in column-major order, mirrored into the upper triangle for the GEMV.
+    hipStream_t s = ctx->stream;
+    TT_HIP(hipMemcpyAsync(Ainv.p, A.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, s));
+    rocblas_handle h = (rocblas_handle)blas;
+    TT_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p) == rocblas_status_success, TT_ERR_HIP,
+             "rocsolver_dpotrf failed");
+    int info = 0;
+    TT_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
+    TT_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p) == rocblas_status_success, TT_ERR_HIP,
+             "rocsolver_dpotri failed");
+    TT_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
+    launch_mirror_lower(Ainv.p, n, s);
+    TT_HIP(hipStreamSynchronize(s));
 }
 
 void Multigrid::setup_operator(double cK_, double cM_) {
