@@ -35,7 +35,7 @@ PEAK_FP64 = 78.6e12        # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_FP32 = 157.3e12       # MI355X dense FP32 matrix, flop/s
 # HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
 # scaled by the counters' reading for a copy kernel of known size); None until measured.
-TRAFFIC_A32 = None
+TRAFFIC_A32 = 20225724  # FETCH 6679.6 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (copy-calibrated)
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)

@@ -85,6 +85,7 @@ def lib():
             "or_hex8_me": (None, [ctypes.c_double] * 3 + [dp]),
             "or_pdefilter_ke": (None, [ctypes.c_double] * 4 + [dp]),
             "or_newmark_consts": (None, [ctypes.c_int, ctypes.c_double, dp]),
+            "or_interp": (None, [ctypes.c_int] + [ctypes.c_double] * 4 + [dp, dp]),
             "or_bc_mask": (None, [P, dp]), "or_load_location": (None, [P, dp]),
             "or_load_intensity": (ctypes.c_double, [P, ctypes.c_int]),
             "or_region": (None, [P, dp]),
@@ -166,6 +167,14 @@ def newmark_consts(ti, dt):
     a = np.zeros(6)
     lib().or_newmark_consts(ti, dt, _p(a))
     return a
+
+
+def interp_c(scheme, vmin, vmax, penal, x):
+    """The C oracle's material interpolation (value, derivative) at a scalar x."""
+    f = ctypes.c_double()
+    df = ctypes.c_double()
+    lib().or_interp(scheme, vmin, vmax, penal, x, ctypes.byref(f), ctypes.byref(df))
+    return f.value, df.value
 
 
 def bc_mask(p):
