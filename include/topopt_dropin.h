@@ -1,0 +1,214 @@
+/* Drop-in C++ classes of the transient topology-optimisation path over the HIP C-ABI
+ * (topopt_gpu.h).
+ *
+ * Same class names, constructor and method signatures, argument meaning and return
+ * convention (PetscErrorCode 0 = ok) as the reference's LinearElasticity.h:29-85,
+ * Filter.h:38-55, PDEFilter.h:32-36, MMA.h:12-53 and TopOpt.h, so that a main program
+ * written against those headers (main.cc) drives the GPU path unchanged.  PETSc is not
+ * linked: the few PETSc names these signatures use are supplied here with minimal
+ * meaning -- Vec is a host array of doubles, DM a structured-grid descriptor that also
+ * carries the GPU context of its mesh, and the options database holds the command line.
+ * MPIIO output objects are accepted and ignored (always pass NULL).
+ *
+ * Differences a caller can observe:
+ *   - Filter supports the PDE filter only (filterT = 2, Filter.cc:60-119 / 122-206);
+ *   - MMA supports one constraint (m = 1) and no restart constructors;
+ *   - WriteRestartFiles() writes nothing; the reduced-order path (reduction = TRUE) is rejected;
+ *   - numerical non-convergence and non-finite values are reported as errors (< 0).
+ */
+#ifndef TOPOPT_DROPIN_H
+#define TOPOPT_DROPIN_H
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "topopt_gpu.h"
+
+/* ---- the PETSc names used by the interfaces */
+typedef double PetscScalar;
+typedef double PetscReal;
+typedef int PetscInt;
+typedef int PetscErrorCode;
+typedef enum { PETSC_FALSE = 0, PETSC_TRUE = 1 } PetscBool;
+
+struct _p_Vec {
+    std::vector<PetscScalar> a;
+};
+typedef _p_Vec* Vec;
+
+struct TTMesh;  // shared GPU context of one node grid
+struct _p_DM {
+    PetscInt nxyz[3];   // nodes per direction (elements for an element DM)
+    PetscScalar xc[6];  // domain box
+    bool elemental;
+    std::shared_ptr<TTMesh> mesh;
+};
+typedef _p_DM* DM;
+
+class MPIIO;  // output writer of the reference; not part of this build (pass NULL)
+
+PetscErrorCode PetscInitialize(int* argc, char*** argv, const char* file, const char* help);
+PetscErrorCode PetscFinalize();
+PetscErrorCode PetscPrintf(int comm, const char* fmt, ...);
+PetscErrorCode PetscOptionsGetInt(void*, const char* pre, const char* name, PetscInt* v, PetscBool* set);
+PetscErrorCode PetscOptionsGetReal(void*, const char* pre, const char* name, PetscReal* v, PetscBool* set);
+PetscErrorCode PetscOptionsGetBool(void*, const char* pre, const char* name, PetscBool* v, PetscBool* set);
+PetscErrorCode PetscOptionsGetString(void*, const char* pre, const char* name, char* v, size_t len, PetscBool* set);
+#define PETSC_COMM_WORLD 0
+#define PETSC_NULL nullptr
+double MPI_Wtime();
+
+PetscErrorCode VecCreateSeq(int comm, PetscInt n, Vec* v);
+PetscErrorCode VecDuplicate(Vec x, Vec* y);
+PetscErrorCode VecDestroy(Vec* v);
+PetscErrorCode VecSet(Vec v, PetscScalar a);
+PetscErrorCode VecScale(Vec v, PetscScalar a);
+PetscErrorCode VecCopy(Vec x, Vec y);
+PetscErrorCode VecGetSize(Vec v, PetscInt* n);
+PetscErrorCode VecGetArray(Vec v, PetscScalar** a);
+PetscErrorCode VecRestoreArray(Vec v, PetscScalar** a);
+PetscErrorCode DMDestroy(DM* dm);
+
+/* The tt_* context shared by the classes built on one mesh (and the one MMA uses). */
+tt_ctx* topopt_dropin_context(DM da_nodes);
+
+class MMA;
+
+/* TopOpt.h / TopOpt.cc:71-518: problem parameters (same option names and defaults),
+   design vectors, meshes. */
+class TopOpt {
+   public:
+    TopOpt(PetscInt nconstraint);
+    TopOpt();
+    ~TopOpt();
+    PetscErrorCode AllocateMMAwithRestart(PetscInt* itr, MMA** mma);
+    PetscErrorCode WriteRestartFiles(PetscInt* itr, MMA* mma);
+
+    PetscScalar xc[6];
+    PetscScalar dx, dy, dz;
+    PetscInt nxyz[3];
+    PetscInt nlvls;
+    PetscScalar nu;
+    PetscScalar V0;
+    DM da_nodes;
+    DM da_elem;
+    PetscInt objective, minmax;
+    PetscBool reduction;
+    PetscBool outputPhysics0, outputPhysics1, outputDesign;
+    PetscScalar omega0;
+    PetscInt basis, boundaries, optimizationRegion, loadloc;
+    PetscInt n, nloc, m;
+    PetscScalar fx, fscale;
+    PetscScalar* gx;
+    PetscScalar Xmin, Xmax, movlim, volfrac;
+    PetscScalar penalK, penalM, Emin, Emax, Rmin, Rmax, Ef, Rf, t0, t1;
+    PetscInt ti, Ts, ncp, tsc;
+    PetscScalar dalpha, dbeta;
+    PetscInt maxItr;
+    PetscScalar rmin;
+    PetscInt filter, sInt, mInt;
+    PetscBool projectionFilter;
+    PetscReal beta, betaFinal, eta;
+    Vec x, xTilde, xPhys, dfdx, xmin, xmax, xold;
+    Vec* dgdx;
+    PetscBool restart, flip;
+
+   private:
+    void Init(PetscInt nconstraint);
+    PetscErrorCode SetUp();
+};
+
+/* LinearElasticity.h:29-85 (LinearElasticity.cc): transient elasticity physics. */
+class LinearElasticity {
+   public:
+    LinearElasticity(DM da_nodes, DM da_elem, PetscScalar nu_in, PetscInt nlvls_in, PetscInt ncp_in, PetscInt tsc_in,
+                     PetscScalar t1_in, PetscInt ti_in, PetscInt objective_in, PetscInt stiffnessInterpolation_in,
+                     PetscInt massInterpolation_in, PetscScalar t0_in, PetscInt boundaries_in, PetscInt loadloc_in,
+                     PetscBool reduction_in, PetscInt basis_in, PetscScalar dalpha_in, PetscScalar dbeta_in,
+                     PetscScalar omega0_in, PetscInt optimizationRegion, PetscScalar V0);
+    ~LinearElasticity();
+    PetscErrorCode ComputeObjectiveSensitivities(PetscScalar* fx, Vec dfdx, Vec xPhys, PetscScalar Emin,
+                                                 PetscScalar Emax, PetscScalar Rmin, PetscScalar Rmax,
+                                                 PetscScalar penalK, PetscScalar penalM, PetscScalar dalpha,
+                                                 PetscScalar dbeta, PetscBool write, MPIIO* output);
+    PetscErrorCode CompleteInitialState(Vec xPhys, PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin,
+                                        PetscScalar Rmax, PetscScalar penalK, PetscScalar penalM, PetscScalar dalpha,
+                                        PetscScalar dbeta, PetscBool write, MPIIO* output2);
+    PetscErrorCode GenerateCheckPoints(PetscBool write, Vec xPhys, MPIIO* output2);
+    PetscErrorCode VolumeConstraint(PetscScalar* gx, Vec dgdx, Vec xPhys, PetscScalar volfrac);
+    PetscErrorCode WriteRestartFiles();
+    Vec GetStateFieldU(PetscInt j);
+    Vec GetStateFieldDU(PetscInt j);
+    Vec GetStateFieldDDU(PetscInt j);
+    DM GetDM() { return da_nodal; }
+    DM da_nodal;
+    PetscInt Getts() { return Ts; }
+
+   private:
+    PetscInt Ts;
+    tt_ctx* ctx;
+    bool forward_done;
+    Vec state[3];
+    PetscErrorCode SetMaterial(PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin, PetscScalar Rmax,
+                               PetscScalar penalK, PetscScalar penalM, PetscScalar dalpha, PetscScalar dbeta);
+    Vec StateField(PetscInt j, int which);
+    double mat[8];
+};
+
+/* PDEFilter.h:32-36 (PDEFilter.cc): Helmholtz-type PDE filter on the node grid. */
+class PDEFilt {
+   public:
+    PDEFilt(DM da_nodes, PetscScalar rmin);
+    ~PDEFilt();
+    PetscErrorCode FilterProject(Vec XX, Vec F);
+    PetscErrorCode Gradients(Vec OS, Vec FS);
+
+   private:
+    tt_ctx* ctx;
+    PetscScalar R;
+};
+
+/* Filter.h:38-55 (Filter.cc): PDE filter, clamp and smooth Heaviside projection. */
+class Filter {
+   public:
+    Filter(DM da_nodes, Vec x, PetscInt filterT, PetscScalar Rin);
+    ~Filter();
+    PetscErrorCode FilterProject(Vec x, Vec xTilde, Vec xPhys, PetscBool projectionFilter, PetscScalar beta,
+                                 PetscScalar eta);
+    PetscErrorCode Gradients(Vec x, Vec xTilde, Vec dfdx, PetscInt m, Vec* dgdx, PetscBool projectionFilter,
+                             PetscScalar beta, PetscScalar eta);
+    PetscBool IncreaseBeta(PetscReal* beta, PetscReal betaFinal, PetscScalar gx, PetscInt itr, PetscReal ch);
+    PetscScalar GetMND(Vec x);
+
+   private:
+    tt_ctx* ctx;
+    PetscInt filterType;
+    PetscScalar R;
+    PDEFilt* pdef;
+};
+
+/* MMA.h:12-53 (MMA.cc): method of moving asymptotes, one constraint. */
+class MMA {
+   public:
+    MMA(PetscInt n, PetscInt m, Vec x);
+    MMA(PetscInt n, PetscInt m, Vec x, PetscScalar* a, PetscScalar* c, PetscScalar* d);
+    ~MMA();
+    PetscErrorCode Update(Vec xval, Vec dfdx, PetscScalar* gx, Vec* dgdx, Vec xmin, Vec xmax);
+    PetscErrorCode Restart(Vec xo1, Vec xo2, Vec U, Vec L);
+    PetscErrorCode SetAsymptotes(PetscScalar init, PetscScalar decrease, PetscScalar increase);
+    PetscErrorCode ConstraintModification(PetscBool conMod);
+    PetscErrorCode SetRobustAsymptotesType(PetscInt val);
+    PetscErrorCode SetOuterMovelimit(PetscScalar Xmin, PetscScalar Xmax, PetscScalar movelim, Vec x, Vec xmin,
+                                     Vec xmax);
+    PetscErrorCode KKTresidual(Vec xval, Vec dfdx, PetscScalar* gx, Vec* dgdx, Vec xmin, Vec xmax, PetscScalar* norm2,
+                               PetscScalar* normInf);
+    PetscScalar DesignChange(Vec x, Vec xold);
+
+   private:
+    PetscInt n, m;
+    tt_mma* h;
+    void Create(PetscScalar a, PetscScalar c, PetscScalar d);
+};
+
+#endif

@@ -95,6 +95,17 @@ const char* tt_last_error(const tt_ctx* ctx); /* ctx may be NULL: last creation 
    interpolation only, nothing is assembled.  nel doubles. */
 int tt_set_density(tt_ctx* ctx, const double* xphys, int on_device);
 
+/* Material and damping arguments of CompleteInitialState / ComputeObjectiveSensitivities
+   (LinearElasticity.h: Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta; used at
+   LinearElasticity.cc:1580-1623 and :185-219).  Replaces the values given to tt_create; the
+   density must be set again afterwards. */
+int tt_set_material(tt_ctx* ctx, double Emin, double Emax, double Rmin, double Rmax, double penalK, double penalM,
+                    double dalpha, double dbeta);
+
+/* PDEFilt construction (PDEFilter.cc:28-191) for filter radius rmin (R = rmin / (2 sqrt 3))
+   on the context's node grid; replaces tt_params.rmin. */
+int tt_set_filter_radius(tt_ctx* ctx, double rmin);
+
 /* One matrix-free operator application y = N.(X (N.u)) + (1-N).u for X in {K, M, A}
    (the MatMult of LinearElasticity.cc:1342-1345); prec 0 = FP64 arrays, 1 = FP32 arrays.
    3*nn entries. */

@@ -38,7 +38,20 @@ def cantilever_case():
                 ke=O.hex8_ke(0.25, 0.25, 0.25, 0.3), me=O.hex8_me(0.25, 0.25, 0.25))
 
 
+def design_loop_case():
+    """The 32x16x16 cantilever design loop (dt 0.01, 100 steps, 20-step pulse, objective 3, PDE filter
+    rmin 0.08 with the default projection beta 0.1 / eta 0.15, volume fraction and start 0.12, move
+    limit 0.2, 5 MMA iterations), solves at rtol 1e-12.  About 90 s on 8 cores."""
+    p = O.cantilever_params(32, 16, 16, nsteps=100, dt=0.01, pulse_steps=20, rtol=1e-12)
+    x, xp, hist = O.design_loop(p, 0.08, 0.12, 0.12, 5, projection=True, beta=0.1, eta=0.15, movlim=0.2)
+    return dict(hist=hist, x=x, xphys=xp, rtol=np.array(1e-12))
+
+
 if __name__ == "__main__":
     out = os.path.join(HERE, "cantilever_8x4x4.npz")
     np.savez_compressed(out, **cantilever_case())
     print("wrote", out, os.path.getsize(out), "bytes")
+    if "--design-loop" in sys.argv:
+        out = os.path.join(HERE, "design_loop_32x16x16.npz")
+        np.savez_compressed(out, **design_loop_case())
+        print("wrote", out, os.path.getsize(out), "bytes")

@@ -25,7 +25,7 @@ EXPORTED = [
     "tt_pde_filter", "tt_filter_project", "tt_filter_gradients", "tt_mnd", "tt_mma_create", "tt_mma_destroy",
     "tt_mma_set_asymptotes", "tt_mma_set_outer_movelimit", "tt_mma_update", "tt_mma_design_change",
     "tt_mma_kkt_residual", "tt_mma_get_asymptotes", "tt_get_stats", "tt_reset_stats", "tt_synchronize",
-    "tt_num_nodes", "tt_num_elements", "tt_time_op_apply",
+    "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
 ]
 
 
@@ -74,6 +74,8 @@ def lib():
             "tt_default_params": [ctypes.POINTER(Params)],
             "tt_create": [ctypes.POINTER(Params), ctypes.POINTER(vp)],
             "tt_set_density": [vp, vp, ip],
+            "tt_set_material": [vp] + [ctypes.c_double] * 8,
+            "tt_set_filter_radius": [vp, ctypes.c_double],
             "tt_op_apply": [vp, ip, ip, vp, vp, ip],
             "tt_complete_initial_state": [vp], "tt_forward": [vp],
             "tt_get_state": [vp, ip, vp, vp, vp, ip],
@@ -157,6 +159,12 @@ class Context:
     def _chk(self, rc):
         if rc != 0:
             raise TTError(rc, lib().tt_last_error(self.h).decode())
+
+    def set_material(self, Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta):
+        self._chk(lib().tt_set_material(self.h, Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta))
+
+    def set_filter_radius(self, rmin):
+        self._chk(lib().tt_set_filter_radius(self.h, rmin))
 
     def set_density(self, x):
         p, dev = _ptr(x)

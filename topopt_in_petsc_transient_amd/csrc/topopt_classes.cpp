@@ -1,0 +1,604 @@
+// Drop-in classes (include/topopt_dropin.h) over the tt_* C-ABI.
+//
+// Each class keeps the reference's signature and call semantics (LinearElasticity.h,
+// Filter.h, PDEFilter.h, MMA.h, TopOpt.h); the arithmetic runs in libtopopt_gpu.so.
+// Vectors are host arrays, so every call moves nel (or 3 nn) doubles over PCIe; the
+// Newmark sweeps and adjoint stay on the device inside the C-ABI calls.
+#include "topopt_dropin.h"
+
+#include <sys/time.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+
+// ------------------------------------------------------------------ minimal PETSc names
+static std::map<std::string, std::string>& options() {
+    static std::map<std::string, std::string> db;
+    return db;
+}
+
+PetscErrorCode PetscInitialize(int* argc, char*** argv, const char*, const char*) {
+    auto& db = options();
+    for (int i = 1; argc && argv && i < *argc; i++) {
+        const char* a = (*argv)[i];
+        if (a[0] != '-' || (a[1] >= '0' && a[1] <= '9') || a[1] == '.') continue;
+        const bool has_val = i + 1 < *argc && ((*argv)[i + 1][0] != '-' || ((*argv)[i + 1][1] >= '0' &&
+                                                                             (*argv)[i + 1][1] <= '9') ||
+                                              (*argv)[i + 1][1] == '.');
+        db[a] = has_val ? (*argv)[++i] : "1";
+    }
+    return 0;
+}
+PetscErrorCode PetscFinalize() { return 0; }
+PetscErrorCode PetscPrintf(int, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vprintf(fmt, ap);
+    va_end(ap);
+    fflush(stdout);
+    return 0;
+}
+static const std::string* opt(const char* name) {
+    auto it = options().find(name);
+    return it == options().end() ? nullptr : &it->second;
+}
+PetscErrorCode PetscOptionsGetInt(void*, const char*, const char* name, PetscInt* v, PetscBool* set) {
+    const std::string* s = opt(name);
+    if (s) *v = (PetscInt)std::strtol(s->c_str(), nullptr, 10);
+    if (set) *set = s ? PETSC_TRUE : PETSC_FALSE;
+    return 0;
+}
+PetscErrorCode PetscOptionsGetReal(void*, const char*, const char* name, PetscReal* v, PetscBool* set) {
+    const std::string* s = opt(name);
+    if (s) *v = std::strtod(s->c_str(), nullptr);
+    if (set) *set = s ? PETSC_TRUE : PETSC_FALSE;
+    return 0;
+}
+PetscErrorCode PetscOptionsGetBool(void*, const char*, const char* name, PetscBool* v, PetscBool* set) {
+    const std::string* s = opt(name);
+    if (s) *v = (*s == "0" || *s == "false" || *s == "FALSE" || *s == "no") ? PETSC_FALSE : PETSC_TRUE;
+    if (set) *set = s ? PETSC_TRUE : PETSC_FALSE;
+    return 0;
+}
+PetscErrorCode PetscOptionsGetString(void*, const char*, const char* name, char* v, size_t len, PetscBool* set) {
+    const std::string* s = opt(name);
+    if (s && len > 0) {
+        std::strncpy(v, s->c_str(), len - 1);
+        v[len - 1] = 0;
+    }
+    if (set) *set = s ? PETSC_TRUE : PETSC_FALSE;
+    return 0;
+}
+double MPI_Wtime() {
+    timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+PetscErrorCode VecCreateSeq(int, PetscInt n, Vec* v) {
+    *v = new _p_Vec;
+    (*v)->a.assign(n, 0.0);
+    return 0;
+}
+PetscErrorCode VecDuplicate(Vec x, Vec* y) { return VecCreateSeq(0, (PetscInt)x->a.size(), y); }
+PetscErrorCode VecDestroy(Vec* v) {
+    delete *v;
+    *v = nullptr;
+    return 0;
+}
+PetscErrorCode VecSet(Vec v, PetscScalar a) {
+    for (auto& e : v->a) e = a;
+    return 0;
+}
+PetscErrorCode VecScale(Vec v, PetscScalar a) {
+    for (auto& e : v->a) e *= a;
+    return 0;
+}
+PetscErrorCode VecCopy(Vec x, Vec y) {
+    y->a = x->a;
+    return 0;
+}
+PetscErrorCode VecGetSize(Vec v, PetscInt* n) {
+    *n = (PetscInt)v->a.size();
+    return 0;
+}
+PetscErrorCode VecGetArray(Vec v, PetscScalar** a) {
+    *a = v->a.data();
+    return 0;
+}
+PetscErrorCode VecRestoreArray(Vec, PetscScalar** a) {
+    *a = nullptr;
+    return 0;
+}
+PetscErrorCode DMDestroy(DM* dm) {
+    delete *dm;
+    *dm = nullptr;
+    return 0;
+}
+
+// ------------------------------------------------------------------ shared GPU context of a mesh
+struct TTMesh {
+    tt_ctx* ctx = nullptr;
+    ~TTMesh() {
+        if (ctx) tt_destroy(ctx);
+    }
+};
+static tt_ctx* g_current = nullptr;  // the context MMA (which gets no DM) works with
+
+static PetscErrorCode report(tt_ctx* ctx, int rc, const char* where) {
+    if (rc != TT_OK) std::fprintf(stderr, "[topopt] %s failed (%d): %s\n", where, rc, tt_last_error(ctx));
+    return rc;
+}
+#define TTCALL(ctx, call)                                   \
+    do {                                                    \
+        const int rc_ = (call);                             \
+        if (rc_ != TT_OK) return report((ctx), rc_, #call); \
+    } while (0)
+
+static DM make_dm(const PetscInt nxyz[3], const PetscScalar xc[6], bool elemental, std::shared_ptr<TTMesh> mesh) {
+    DM d = new _p_DM;
+    for (int i = 0; i < 3; i++) d->nxyz[i] = nxyz[i];
+    for (int i = 0; i < 6; i++) d->xc[i] = xc[i];
+    d->elemental = elemental;
+    d->mesh = std::move(mesh);
+    return d;
+}
+
+// filter-only context for a mesh that has no physics yet (one step, default material)
+static tt_ctx* mesh_context(DM da) {
+    if (!da->mesh) da->mesh = std::make_shared<TTMesh>();
+    if (!da->mesh->ctx) {
+        tt_params p;
+        tt_default_params(&p);
+        p.nelx = da->nxyz[0] - 1;
+        p.nely = da->nxyz[1] - 1;
+        p.nelz = da->nxyz[2] - 1;
+        for (int i = 0; i < 6; i++) p.xc[i] = da->xc[i];
+        p.nsteps = 1;
+        p.nlvls = 1;
+        tt_ctx* c = nullptr;
+        if (report(nullptr, tt_create(&p, &c), "tt_create") != TT_OK) std::exit(1);
+        da->mesh->ctx = c;
+    }
+    g_current = da->mesh->ctx;
+    return da->mesh->ctx;
+}
+tt_ctx* topopt_dropin_context(DM da_nodes) { return da_nodes && da_nodes->mesh ? da_nodes->mesh->ctx : nullptr; }
+
+// ------------------------------------------------------------------ TopOpt (TopOpt.cc:71-518)
+TopOpt::TopOpt(PetscInt nconstraint) { Init(nconstraint); }
+TopOpt::TopOpt() { Init(1); }
+void TopOpt::Init(PetscInt nconstraint) {
+    m = nconstraint;
+    gx = nullptr;
+    x = xTilde = xPhys = dfdx = xmin = xmax = xold = nullptr;
+    dgdx = nullptr;
+    da_nodes = da_elem = nullptr;
+    fx = 0.0;
+    fscale = 1.0;
+    restart = PETSC_FALSE;
+    flip = PETSC_TRUE;
+    if (SetUp() != 0) std::exit(1);
+}
+TopOpt::~TopOpt() {
+    for (Vec* v : {&x, &xTilde, &xPhys, &dfdx, &xmin, &xmax, &xold})
+        if (*v) VecDestroy(v);
+    if (dgdx) {
+        for (PetscInt i = 0; i < m; i++) VecDestroy(&dgdx[i]);
+        delete[] dgdx;
+    }
+    delete[] gx;
+    if (da_nodes) DMDestroy(&da_nodes);
+    if (da_elem) DMDestroy(&da_elem);
+}
+
+PetscErrorCode TopOpt::SetUp() {
+    PetscBool flg;
+    // defaults as TopOpt.cc:71-170
+    boundaries = 0;
+    PetscOptionsGetInt(NULL, NULL, "-boundaries", &boundaries, &flg);
+    if (boundaries == 1) {
+        nxyz[0] = 65; nxyz[1] = 33; nxyz[2] = 33;
+        xc[0] = 0.0; xc[1] = 40.0; xc[2] = 0.0; xc[3] = 20.0; xc[4] = 0.0; xc[5] = 20.0;
+        loadloc = 1;
+    } else {
+        nxyz[0] = 97; nxyz[1] = 33; nxyz[2] = 33;
+        xc[0] = 0.0; xc[1] = 60.0; xc[2] = 0.0; xc[3] = 20.0; xc[4] = 0.0; xc[5] = 20.0;
+        loadloc = 0;
+    }
+    nu = 0.3;
+    nlvls = 4;
+    volfrac = 0.3;
+    V0 = volfrac;
+    maxItr = 10;
+    rmin = 1.5;
+    penalK = 3.0;
+    penalM = 1.0;
+    Emax = 1e3;
+    Rmax = 4.16666666667e-4;
+    filter = 1;
+    sInt = 1;
+    mInt = 0;
+    Xmin = 0.0;
+    Xmax = 1.0;
+    movlim = 0.2;
+    Rf = 1e-3;
+    Ef = 1e-9;
+    ti = 1;
+    t0 = 0.0;
+    t1 = 50;
+    ncp = 2;
+    tsc = 25;
+    dalpha = 1e-4;
+    dbeta = 1e-4;
+    objective = 0;
+    minmax = -1;
+    optimizationRegion = 0;
+    projectionFilter = PETSC_TRUE;
+    beta = 0.1;
+    betaFinal = 48;
+    eta = 0.15;
+    reduction = PETSC_FALSE;
+    basis = 5;
+    omega0 = 1.0;
+    outputPhysics0 = outputPhysics1 = outputDesign = PETSC_FALSE;
+    // mesh (SetUpMESH, TopOpt.cc:175-330)
+    PetscOptionsGetInt(NULL, NULL, "-nx", &nxyz[0], &flg);
+    PetscOptionsGetInt(NULL, NULL, "-ny", &nxyz[1], &flg);
+    PetscOptionsGetInt(NULL, NULL, "-nz", &nxyz[2], &flg);
+    PetscOptionsGetReal(NULL, NULL, "-xcmin", &xc[0], &flg);
+    PetscOptionsGetReal(NULL, NULL, "-xcmax", &xc[1], &flg);
+    PetscOptionsGetReal(NULL, NULL, "-ycmin", &xc[2], &flg);
+    PetscOptionsGetReal(NULL, NULL, "-ycmax", &xc[3], &flg);
+    PetscOptionsGetReal(NULL, NULL, "-zcmin", &xc[4], &flg);
+    PetscOptionsGetReal(NULL, NULL, "-zcmax", &xc[5], &flg);
+    PetscOptionsGetInt(NULL, NULL, "-nlvls", &nlvls, &flg);
+    const PetscInt div = 1 << (nlvls - 1);
+    for (int d = 0; d < 3; d++)
+        if ((nxyz[d] - 1) % div != 0) {
+            PetscPrintf(PETSC_COMM_WORLD, "MESH DIMENSION NOT COMPATIBLE WITH NUMBER OF MULTIGRID LEVELS!\n");
+            return TT_ERR_ARG;
+        }
+    dx = (xc[1] - xc[0]) / (nxyz[0] - 1);
+    dy = (xc[3] - xc[2]) / (nxyz[1] - 1);
+    dz = (xc[5] - xc[4]) / (nxyz[2] - 1);
+    auto mesh = std::make_shared<TTMesh>();
+    da_nodes = make_dm(nxyz, xc, false, mesh);
+    const PetscInt nel[3] = {nxyz[0] - 1, nxyz[1] - 1, nxyz[2] - 1};
+    da_elem = make_dm(nel, xc, true, mesh);
+    // optimisation parameters (SetUpOPT, TopOpt.cc:340-470)
+    PetscOptionsGetInt(NULL, NULL, "-boundaries", &boundaries, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-loadloc", &loadloc, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-filter", &filter, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-rmin", &rmin, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-projectionFilter", &projectionFilter, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-beta", &beta, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-betaFinal", &betaFinal, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-eta", &eta, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-volfrac", &volfrac, &flg);
+    V0 = volfrac;
+    PetscOptionsGetReal(NULL, NULL, "-Xmax", &Xmax, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-Xmin", &Xmin, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-maxItr", &maxItr, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-movlim", &movlim, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-optimizationRegion", &optimizationRegion, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-V0", &V0, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-nu", &nu, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-penalM", &penalM, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-penalK", &penalK, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-Rf", &Rf, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-Ef", &Ef, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-Emax", &Emax, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-Rmax", &Rmax, &flg);
+    Emin = Ef * Emax;
+    Rmin = Rf * Rmax;
+    PetscOptionsGetReal(NULL, NULL, "-dalph", &dalpha, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-dbeta", &dbeta, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-massInterpolation", &mInt, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-sInt", &sInt, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-ti", &ti, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-t0", &t0, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-t1", &t1, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-ncp", &ncp, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-tsc", &tsc, &flg);
+    Ts = ncp * tsc + 1;
+    PetscOptionsGetInt(NULL, NULL, "-objective", &objective, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-minmax", &minmax, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-reduction", &reduction, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-basis", &basis, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-omega0", &omega0, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-outputPhysics0", &outputPhysics0, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-outputPhysics1", &outputPhysics1, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-outputDesign", &outputDesign, &flg);
+    n = nel[0] * nel[1] * nel[2];
+    nloc = n;
+    gx = new PetscScalar[m];
+    for (PetscInt i = 0; i < m; i++) gx[i] = 0.0;
+    // design vectors initialised to V0 (TopOpt.cc:495-515)
+    VecCreateSeq(0, n, &xPhys);
+    VecDuplicate(xPhys, &x);
+    VecDuplicate(xPhys, &xTilde);
+    VecSet(x, V0);
+    VecSet(xTilde, V0);
+    VecSet(xPhys, V0);
+    VecDuplicate(x, &dfdx);
+    dgdx = new Vec[m];
+    for (PetscInt i = 0; i < m; i++) VecDuplicate(x, &dgdx[i]);
+    VecDuplicate(x, &xmin);
+    VecDuplicate(x, &xmax);
+    VecDuplicate(x, &xold);
+    VecSet(xold, V0);
+    PetscPrintf(PETSC_COMM_WORLD, "# nodes (%d,%d,%d), elements %d, dofs %d, nlvls %d, Ts %d, volfrac %g, rmin %g\n",
+                nxyz[0], nxyz[1], nxyz[2], n, 3 * nxyz[0] * nxyz[1] * nxyz[2], nlvls, Ts, volfrac, rmin);
+    return 0;
+}
+
+// MMA allocation with a = 0, c = 1000, d = 0 (TopOpt.cc:527-531); restart files are not read
+PetscErrorCode TopOpt::AllocateMMAwithRestart(PetscInt* itr, MMA** mma) {
+    PetscBool flg;
+    PetscOptionsGetBool(NULL, NULL, "-restart", &restart, &flg);
+    if (restart && flg) {
+        PetscPrintf(PETSC_COMM_WORLD, "# restart files are not supported by this build: starting from V0\n");
+        restart = PETSC_FALSE;
+    }
+    std::vector<PetscScalar> a(m, 0.0), c(m, 1000.0), d(m, 0.0);
+    *itr = 0;
+    *mma = new MMA(n, m, x, a.data(), c.data(), d.data());
+    return 0;
+}
+PetscErrorCode TopOpt::WriteRestartFiles(PetscInt*, MMA*) { return 0; }
+
+// ------------------------------------------------------------------ LinearElasticity
+LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt nlvls_in, PetscInt ncp_in,
+                                   PetscInt tsc_in, PetscScalar t1_in, PetscInt ti_in, PetscInt objective_in,
+                                   PetscInt sInt_in, PetscInt mInt_in, PetscScalar t0_in, PetscInt boundaries_in,
+                                   PetscInt loadloc_in, PetscBool reduction_in, PetscInt, PetscScalar dalpha_in,
+                                   PetscScalar dbeta_in, PetscScalar, PetscInt optimizationRegion, PetscScalar) {
+    da_nodal = da_nodes;
+    ctx = nullptr;
+    forward_done = false;
+    for (auto& s : state) s = nullptr;
+    for (auto& v : mat) v = std::nan("");
+    Ts = ncp_in * tsc_in + 1;  // LinearElasticity.cc:96
+    if (reduction_in) {
+        std::fprintf(stderr, "[topopt] the reduced-order (-reduction) path is not part of this build\n");
+        std::exit(1);
+    }
+    tt_params p;
+    tt_default_params(&p);
+    p.nelx = da_nodes->nxyz[0] - 1;
+    p.nely = da_nodes->nxyz[1] - 1;
+    p.nelz = da_nodes->nxyz[2] - 1;
+    for (int i = 0; i < 6; i++) p.xc[i] = da_nodes->xc[i];
+    p.nu = nu_in;
+    p.nlvls = nlvls_in;
+    p.nsteps = Ts - 1;
+    p.t1 = t1_in - t0_in;
+    p.ti = ti_in;
+    p.objective = objective_in;
+    p.sInt = sInt_in;
+    p.mInt = mInt_in;
+    p.boundaries = boundaries_in;
+    p.loadloc = loadloc_in;
+    p.region = optimizationRegion;
+    p.dalpha = dalpha_in;
+    p.dbeta = dbeta_in;
+    // options of this build (no reference counterpart): precision, state slots, solver, load signal
+    PetscBool flg;
+    PetscOptionsGetInt(NULL, NULL, "-precision", &p.precision, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-slots", &p.checkpoint_slots, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-ksp_rtol", &p.rtol, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-ksp_max_it", &p.maxit, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-cheb_degree", &p.cheb_degree, &flg);
+    p.loadsignal = 0;  // the reference's Gaussian-cosine signal (LinearElasticity.cc:593-615) by default
+    PetscOptionsGetInt(NULL, NULL, "-loadsignal", &p.loadsignal, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-pulse_steps", &p.pulse_steps, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-filter_rtol", &p.filter_rtol, &flg);
+    if (report(nullptr, tt_create(&p, &ctx), "tt_create") != TT_OK) std::exit(1);
+    if (!da_nodes->mesh) da_nodes->mesh = std::make_shared<TTMesh>();
+    if (da_nodes->mesh->ctx) tt_destroy(da_nodes->mesh->ctx);  // a filter-only context is superseded
+    da_nodes->mesh->ctx = ctx;
+    g_current = ctx;
+}
+LinearElasticity::~LinearElasticity() {
+    for (auto& s : state)
+        if (s) VecDestroy(&s);
+}
+
+PetscErrorCode LinearElasticity::SetMaterial(PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin, PetscScalar Rmax,
+                                             PetscScalar penalK, PetscScalar penalM, PetscScalar dalpha,
+                                             PetscScalar dbeta) {
+    const double m[8] = {Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta};
+    if (std::memcmp(m, mat, sizeof(m)) != 0) {
+        TTCALL(ctx, tt_set_material(ctx, Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta));
+        std::memcpy(mat, m, sizeof(m));
+    }
+    return 0;
+}
+
+// LinearElasticity.cc:956-1206
+PetscErrorCode LinearElasticity::CompleteInitialState(Vec xPhys, PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin,
+                                                      PetscScalar Rmax, PetscScalar penalK, PetscScalar penalM,
+                                                      PetscScalar dalpha, PetscScalar dbeta, PetscBool, MPIIO*) {
+    PetscErrorCode ierr = SetMaterial(Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta);
+    if (ierr) return ierr;
+    TTCALL(ctx, tt_set_density(ctx, xPhys->a.data(), 0));
+    TTCALL(ctx, tt_complete_initial_state(ctx));
+    forward_done = false;
+    return 0;
+}
+
+// LinearElasticity.cc:1208-1280 (binomial checkpoints in device memory instead of files)
+PetscErrorCode LinearElasticity::GenerateCheckPoints(PetscBool, Vec, MPIIO*) {
+    TTCALL(ctx, tt_forward(ctx));
+    forward_done = true;
+    return 0;
+}
+
+// LinearElasticity.cc:1625-1908; dfdx is overwritten (the reference zeroes it first, :1649)
+PetscErrorCode LinearElasticity::ComputeObjectiveSensitivities(PetscScalar* fx, Vec dfdx, Vec xPhys,
+                                                               PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin,
+                                                               PetscScalar Rmax, PetscScalar penalK,
+                                                               PetscScalar penalM, PetscScalar dalpha,
+                                                               PetscScalar dbeta, PetscBool, MPIIO*) {
+    const double m[8] = {Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta};
+    if (std::memcmp(m, mat, sizeof(m)) != 0) {  // new material: redo the initial state as the reference would
+        PetscErrorCode ierr = CompleteInitialState(xPhys, Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta,
+                                                   PETSC_FALSE, nullptr);
+        if (ierr) return ierr;
+    }
+    if (!forward_done) {
+        TTCALL(ctx, tt_forward(ctx));
+        forward_done = true;
+    }
+    TTCALL(ctx, tt_objective_gradient(ctx, fx, dfdx->a.data(), 0));
+    return 0;
+}
+
+// LinearElasticity.cc:1283-1296
+PetscErrorCode LinearElasticity::VolumeConstraint(PetscScalar* gx, Vec dgdx, Vec xPhys, PetscScalar volfrac) {
+    TTCALL(ctx, tt_volume_constraint(ctx, xPhys->a.data(), volfrac, gx, dgdx->a.data(), 0));
+    return 0;
+}
+PetscErrorCode LinearElasticity::WriteRestartFiles() { return 0; }
+
+Vec LinearElasticity::StateField(PetscInt j, int which) {
+    const long nd = 3 * tt_num_nodes(ctx);
+    if (!state[which]) VecCreateSeq(0, (PetscInt)nd, &state[which]);
+    double* p[3] = {nullptr, nullptr, nullptr};
+    p[which] = state[which]->a.data();
+    if (report(ctx, tt_get_state(ctx, j, p[0], p[1], p[2], 0), "tt_get_state") != TT_OK) return nullptr;
+    return state[which];
+}
+Vec LinearElasticity::GetStateFieldU(PetscInt j) { return StateField(j, 0); }
+Vec LinearElasticity::GetStateFieldDU(PetscInt j) { return StateField(j, 1); }
+Vec LinearElasticity::GetStateFieldDDU(PetscInt j) { return StateField(j, 2); }
+
+// ------------------------------------------------------------------ PDEFilt (PDEFilter.cc)
+PDEFilt::PDEFilt(DM da_nodes, PetscScalar rmin) : R(rmin) {
+    ctx = mesh_context(da_nodes);
+    if (report(ctx, tt_set_filter_radius(ctx, rmin), "tt_set_filter_radius") != TT_OK) std::exit(1);
+}
+PDEFilt::~PDEFilt() {}
+PetscErrorCode PDEFilt::FilterProject(Vec XX, Vec F) {
+    TTCALL(ctx, tt_pde_filter(ctx, XX->a.data(), F->a.data(), 0));
+    return 0;
+}
+PetscErrorCode PDEFilt::Gradients(Vec OS, Vec FS) {  // self-adjoint (PDEFilter.cc:219-224)
+    TTCALL(ctx, tt_pde_filter(ctx, OS->a.data(), FS->a.data(), 0));
+    return 0;
+}
+
+// ------------------------------------------------------------------ Filter (Filter.cc)
+Filter::Filter(DM da_nodes, Vec, PetscInt filterT, PetscScalar Rin) : filterType(filterT), R(Rin) {
+    if (filterT != 2) {
+        std::fprintf(stderr, "[topopt] only the PDE filter (-filter 2) is part of this build (got %d)\n", filterT);
+        std::exit(1);
+    }
+    pdef = new PDEFilt(da_nodes, Rin);
+    ctx = mesh_context(da_nodes);
+}
+Filter::~Filter() { delete pdef; }
+
+// Filter.cc:60-119: PDE filter, clamp to [0, 1], optional smooth Heaviside projection
+PetscErrorCode Filter::FilterProject(Vec x, Vec xTilde, Vec xPhys, PetscBool projectionFilter, PetscScalar beta,
+                                     PetscScalar eta) {
+    TTCALL(ctx, tt_filter_project(ctx, x->a.data(), xTilde->a.data(), xPhys->a.data(), projectionFilter ? 1 : 0,
+                                  beta, eta, 0));
+    return 0;
+}
+
+// Filter.cc:122-206: projection chain rule, then the PDE filter transpose, for dfdx and each dgdx
+PetscErrorCode Filter::Gradients(Vec, Vec xTilde, Vec dfdx, PetscInt m, Vec* dgdx, PetscBool projectionFilter,
+                                 PetscScalar beta, PetscScalar eta) {
+    if (m < 1) return TT_ERR_ARG;
+    TTCALL(ctx, tt_filter_gradients(ctx, xTilde->a.data(), dfdx->a.data(), dgdx[0]->a.data(),
+                                    projectionFilter ? 1 : 0, beta, eta, 0));
+    for (PetscInt i = 1; i < m; i++) {
+        std::vector<double> scratch(dgdx[i]->a.size(), 0.0);
+        TTCALL(ctx, tt_filter_gradients(ctx, xTilde->a.data(), scratch.data(), dgdx[i]->a.data(),
+                                        projectionFilter ? 1 : 0, beta, eta, 0));
+    }
+    return 0;
+}
+
+// Filter.cc:274-300
+PetscBool Filter::IncreaseBeta(PetscReal* beta, PetscReal betaFinal, PetscScalar gx, PetscInt itr, PetscReal ch) {
+    PetscBool change = PETSC_FALSE;
+    if ((ch < 0.01 || itr % 10 == 0) && beta[0] < betaFinal && gx <= 0.0) {
+        change = PETSC_TRUE;
+        beta[0] = beta[0] < 7 ? beta[0] + 1 : beta[0] * 1.2;
+        if (beta[0] > betaFinal) {
+            beta[0] = betaFinal;
+            change = PETSC_FALSE;
+        }
+        PetscPrintf(PETSC_COMM_WORLD, "# Beta has been increased to: %f\n", beta[0]);
+    }
+    return change;
+}
+
+// Filter.cc:208-227
+PetscScalar Filter::GetMND(Vec x) {
+    double mnd = 0.0;
+    if (report(ctx, tt_mnd(ctx, x->a.data(), &mnd, 0), "tt_mnd") != TT_OK) return std::nan("");
+    return mnd;
+}
+
+// ------------------------------------------------------------------ MMA (MMA.cc)
+void MMA::Create(PetscScalar a, PetscScalar c, PetscScalar d) {
+    if (m != 1) {
+        std::fprintf(stderr, "[topopt] the GPU MMA handles one constraint (m = %d requested)\n", m);
+        std::exit(1);
+    }
+    if (!g_current || n != tt_num_elements(g_current)) {
+        std::fprintf(stderr, "[topopt] MMA needs the mesh context of %d design variables\n", n);
+        std::exit(1);
+    }
+    if (report(g_current, tt_mma_create(g_current, a, c, d, &h), "tt_mma_create") != TT_OK) std::exit(1);
+}
+MMA::MMA(PetscInt nn, PetscInt mm, Vec) : n(nn), m(mm), h(nullptr) { Create(0.0, 1000.0, 0.0); }
+MMA::MMA(PetscInt nn, PetscInt mm, Vec, PetscScalar* a, PetscScalar* c, PetscScalar* d) : n(nn), m(mm), h(nullptr) {
+    Create(a[0], c[0], d[0]);
+}
+MMA::~MMA() {
+    if (h) tt_mma_destroy(h);
+}
+PetscErrorCode MMA::Update(Vec xval, Vec dfdx, PetscScalar* gx, Vec* dgdx, Vec xmin, Vec xmax) {
+    TTCALL(g_current, tt_mma_update(h, xval->a.data(), dfdx->a.data(), gx, dgdx[0]->a.data(), xmin->a.data(),
+                                    xmax->a.data(), 0));
+    return 0;
+}
+PetscErrorCode MMA::Restart(Vec, Vec, Vec U, Vec L) {
+    double lam = 0.0;
+    TTCALL(g_current, tt_mma_get_asymptotes(h, L->a.data(), U->a.data(), &lam));
+    return TT_ERR_STATE;  // previous iterates are not exported: no restart files in this build
+}
+PetscErrorCode MMA::SetAsymptotes(PetscScalar init, PetscScalar decrease, PetscScalar increase) {
+    TTCALL(g_current, tt_mma_set_asymptotes(h, init, decrease, increase));
+    return 0;
+}
+PetscErrorCode MMA::ConstraintModification(PetscBool conMod) { return conMod ? TT_ERR_ARG : 0; }
+PetscErrorCode MMA::SetRobustAsymptotesType(PetscInt val) { return val == 0 ? 0 : TT_ERR_ARG; }
+PetscErrorCode MMA::SetOuterMovelimit(PetscScalar Xmin, PetscScalar Xmax, PetscScalar movelim, Vec x, Vec xmin,
+                                      Vec xmax) {
+    TTCALL(g_current, tt_mma_set_outer_movelimit(h, Xmin, Xmax, movelim, x->a.data(), xmin->a.data(),
+                                                 xmax->a.data(), 0));
+    return 0;
+}
+PetscErrorCode MMA::KKTresidual(Vec xval, Vec dfdx, PetscScalar* gx, Vec* dgdx, Vec xmin, Vec xmax,
+                                PetscScalar* norm2, PetscScalar* normInf) {
+    TTCALL(g_current, tt_mma_kkt_residual(h, xval->a.data(), dfdx->a.data(), gx, dgdx[0]->a.data(), xmin->a.data(),
+                                          xmax->a.data(), norm2, normInf, 0));
+    return 0;
+}
+PetscScalar MMA::DesignChange(Vec x, Vec xold) {
+    double ch = 0.0;
+    if (report(g_current, tt_mma_design_change(h, x->a.data(), xold->a.data(), &ch, 0), "tt_mma_design_change") !=
+        TT_OK)
+        return std::nan("");
+    return ch;
+}

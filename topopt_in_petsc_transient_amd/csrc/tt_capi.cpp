@@ -158,13 +158,19 @@ Context::Context(const tt_params& p) : prm(p) {
     st_cur.alloc(3 * nd);
     st_next.alloc(3 * nd);
 
-    // PDE filter (PDEFilter.cc:28-191): R = rmin / (2 sqrt 3), KF on the same node grid
+    // PDE filter (PDEFilter.cc:28-191) on the same node grid
     fg = g;
-    const double R = p.rmin / 2.0 / std::sqrt(3.0);
-    hex8_helmholtz(g.hx, g.hy, g.hz, R, KF);
     KFd.alloc(64);
-    TT_HIP(hipMemcpy(KFd.p, KF, sizeof(KF), hipMemcpyHostToDevice));
     for (DBuf<double>* b : {&fdinv, &fU, &frhs, &fr, &fz, &fp, &fq}) b->alloc(nn);
+    set_filter_radius(p.rmin);
+}
+
+// R = rmin / (2 sqrt 3) (PDEFilter.cc:32); element matrix KF and its assembled diagonal
+void Context::set_filter_radius(double rmin) {
+    prm.rmin = rmin;
+    const double R = rmin / 2.0 / std::sqrt(3.0);
+    hex8_helmholtz(g.hx, g.hy, g.hz, R, KF);
+    TT_HIP(hipMemcpyAsync(KFd.p, KF, sizeof(KF), hipMemcpyHostToDevice, stream));
     launch_filter_diag(fg, KFd.p, fdinv.p, stream);
     sync();
 }
@@ -428,6 +434,30 @@ int tt_set_density(tt_ctx* ctx, const double* xphys, int on_device) {
     C.density_set = true;
     C.initial_done = false;
     C.forward_done = false;
+    TT_API_END
+}
+
+int tt_set_material(tt_ctx* ctx, double Emin, double Emax, double Rmin, double Rmax, double penalK, double penalM,
+                    double dalpha, double dbeta) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(std::isfinite(Emin) && std::isfinite(Emax) && std::isfinite(Rmin) && std::isfinite(Rmax) &&
+                 std::isfinite(penalK) && std::isfinite(penalM) && std::isfinite(dalpha) && std::isfinite(dbeta),
+             TT_ERR_ARG, "non-finite material parameter");
+    tt_params& p = C.prm;
+    p.Emin = Emin; p.Emax = Emax; p.Rmin = Rmin; p.Rmax = Rmax;
+    p.penalK = penalK; p.penalM = penalM; p.dalpha = dalpha; p.dbeta = dbeta;
+    C.cK_A = 1.0 + C.a[2] * dbeta;
+    C.cM_A = C.a[5] + C.a[2] * dalpha;
+    C.density_set = false;
+    C.initial_done = false;
+    C.forward_done = false;
+    TT_API_END
+}
+
+int tt_set_filter_radius(tt_ctx* ctx, double rmin) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(std::isfinite(rmin) && rmin >= 0.0, TT_ERR_ARG, "filter radius must be finite and >= 0");
+    C.set_filter_radius(rmin);
     TT_API_END
 }
 

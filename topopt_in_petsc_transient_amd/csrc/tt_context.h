@@ -138,6 +138,7 @@ struct Context {
     double load_intensity(int step) const;
 
     // filter
+    void set_filter_radius(double rmin);
     void pde_filter(const double* in_dev, double* out_dev);
 };
 
