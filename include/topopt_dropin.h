@@ -8,7 +8,7 @@
  * linked: the few PETSc names these signatures use are supplied here with minimal
  * meaning -- Vec is a host array of doubles, DM a structured-grid descriptor that also
  * carries the GPU context of its mesh, and the options database holds the command line.
- * MPIIO output objects are accepted and ignored (always pass NULL).
+ * MPIIO output objects can be created and are accepted, but write nothing.
  *
  * Differences a caller can observe:
  *   - Filter supports the PDE filter only (filterT = 2, Filter.cc:60-119 / 122-206);
@@ -26,6 +26,7 @@
 #include "topopt_gpu.h"
 
 /* ---- the PETSc names used by the interfaces */
+#define PETSC_MAX_PATH_LEN 4096
 typedef double PetscScalar;
 typedef double PetscReal;
 typedef int PetscInt;
@@ -46,7 +47,17 @@ struct _p_DM {
 };
 typedef _p_DM* DM;
 
-class MPIIO;  // output writer of the reference; not part of this build (pass NULL)
+/* MPIIO.h:35-46: the output writer.  Field output is not part of this build: the object can be
+   constructed (so main.cc compiles unchanged), prints one notice, and WriteVTK writes nothing. */
+class MPIIO {
+   public:
+    MPIIO(DM da_nodes, int nPfields, std::string pnames, int nCfields, std::string cnames, std::string filename);
+    ~MPIIO();
+    PetscErrorCode WriteVTK(DM da_nodes, Vec x, Vec xTilde, Vec xPhys, PetscInt itr);
+    PetscErrorCode WriteVTK(DM da_nodes, Vec U, Vec x, Vec xTilde, Vec xPhys, PetscInt itr);
+    PetscErrorCode WriteVTK(DM da_nodes, Vec U, Vec xPhys, PetscInt itr);
+    PetscErrorCode WriteVTK(DM da_nodes, Vec U, Vec DU, Vec DDU, Vec x, Vec xtilde, Vec xPhys, PetscInt itr);
+};
 
 PetscErrorCode PetscInitialize(int* argc, char*** argv, const char* file, const char* help);
 PetscErrorCode PetscFinalize();

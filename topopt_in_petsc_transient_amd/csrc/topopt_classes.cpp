@@ -120,6 +120,17 @@ PetscErrorCode DMDestroy(DM* dm) {
     return 0;
 }
 
+// ------------------------------------------------------------------ output writer (not part of this build)
+MPIIO::MPIIO(DM, int, std::string, int, std::string, std::string filename) {
+    std::fprintf(stderr, "[topopt] field output (%s) is not part of this build: nothing is written\n",
+                 filename.c_str());
+}
+MPIIO::~MPIIO() {}
+PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, Vec, PetscInt) { return 0; }
+PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, Vec, Vec, PetscInt) { return 0; }
+PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, PetscInt) { return 0; }
+PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, Vec, Vec, Vec, Vec, PetscInt) { return 0; }
+
 // ------------------------------------------------------------------ shared GPU context of a mesh
 struct TTMesh {
     tt_ctx* ctx = nullptr;
