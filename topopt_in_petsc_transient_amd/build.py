@@ -20,7 +20,9 @@ LIB_SOURCES = ["tt_kernels.hip", "tt_kernels2.hip", "tt_element.cpp", "tt_solver
                "tt_capi.cpp"]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
              # no packed FP32 instructions (v_pk_*_f32) in any generated code
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # matrix-core results straight into VGPRs (no accumulation-register copies in the tile loop)
+             "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 
 
 def _compile(src):

@@ -56,9 +56,13 @@ struct Multigrid {
     void setup_operator(double cK_, double cM_);
     // z = B r (FP64 in and out)
     void apply(const double* r, double* z);
+    // z = B r as the flexible-CG preconditioner, also leaving the partials of z.r and z.r_prev
+    // (r_prev ignored when first) in the reducer's PB_DOTS region
+    void apply_fcg(const double* r, const double* rp, double* z, bool first);
 
    private:
-    void vcycle(int l);
+    void vcycle(int l, const double* r64 = nullptr, double* z64 = nullptr, const double* rp64 = nullptr,
+                bool first = false);
     void coarse_solve();
     double power_lmax(int l, int iters);
     void assemble_coarse_inverse();

@@ -182,6 +182,13 @@ void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, c
 void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s);
 void launch_copy(const void* src, void* dst, long n, bool src64, bool dst64, hipStream_t s);
 void launch_cheb_first(const void* b, const void* dinv, double c1, void* x, long n, bool fp64, hipStream_t s);
+// b = (T) r and x = c1 D^-1 b (finest level, r in FP64)
+void launch_cheb_first_from64(const double* r, void* b, const void* dinv, double c1, void* x, long n, bool fp64,
+                              hipStream_t s);
+// last post-smoothing step of the finest level: z (FP64) = x + dn, partials of z.r and z.r_prev into PB_DOTS
+void launch_cheb_last(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
+                      const void* dinv, double c0, double c1, double* z, const double* r, const double* rp, bool first,
+                      bool fp64, Reducer& red, hipStream_t s);
 void launch_restrict(const Grid& fine, const Grid& coarse, const void* rf, const uint8_t* cmask, void* bc,
                      bool fp64, hipStream_t s);
 void launch_prolong_add(const Grid& fine, const Grid& coarse, const void* xc, const uint8_t* fmask, void* xf,

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
     template <class S, typename V>                                                                     \
     __device__ void operator()(bool in, int n, uint8_t cl, const V (&yK)[3], const V (&yM)[3],          \
                                const S& st) const {                                                     \
-        double acc = 0.0;                                                                              \
+        double acc[2] = {0.0, 0.0};                                                                    \
         if (in) put(n, cl, yK, yM, st, acc);                                                           \
         done(acc);                                                                                     \
     }
@@ -305,11 +305,11 @@ template <typename T>
 struct EpiBC {  // y = A u on free dofs, u on clamped
     T* y;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double&) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
     }
-    __device__ void done(double) const {}
+    __device__ void done(double (&)[2]) const {}
     TT_EPI_CALL
 };
 
@@ -318,17 +318,17 @@ struct EpiBCDot {  // y = A u (BC form) and partials of u.y
     T* y;
     double* part;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double& acc) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&acc)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const T u = st.k(3 * n + c);
             const T v = ((cl >> c) & 1) ? u : yK[c] + yM[c];
             y[3 * n + c] = v;
-            acc += (double)u * (double)v;
+            acc[0] += (double)u * (double)v;
         }
     }
-    __device__ void done(double acc) const {
-        double a[1] = {acc};
+    __device__ void done(double (&acc)[2]) const {
+        double a[1] = {acc[0]};
         write_partials<1>(a, part);
     }
     TT_EPI_CALL
@@ -339,14 +339,14 @@ struct EpiResid {  // r = b - A x
     const T* b;
     T* r;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double&) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const T ax = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
             r[3 * n + c] = b[3 * n + c] - ax;
         }
     }
-    __device__ void done(double) const {}
+    __device__ void done(double (&)[2]) const {}
     TT_EPI_CALL
 };
 
@@ -359,7 +359,7 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
     T* xn;
     T* dn;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double&) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -372,7 +372,39 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
             if (dn) dn[i] = dd;
         }
     }
-    __device__ void done(double) const {}
+    __device__ void done(double (&)[2]) const {}
+    TT_EPI_CALL
+};
+
+// Last post-smoothing step of the finest level: z = x + c0 d + c1 D^-1 (b - A x) written in FP64
+// as the preconditioned residual of flexible CG, with partials of z.r and z.r_prev.
+template <typename T>
+struct EpiChebOut {
+    const T* b;
+    const T* d;
+    const T* dinv;
+    T c0, c1;
+    double* z;
+    const double* r;
+    const double* rp;
+    int first;
+    double* part;
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&acc)[2]) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            const T x = st.k(i);
+            const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
+            T dd = c1 * dinv[i] * (b[i] - ax);
+            if (d) dd = fma(c0, d[i], dd);
+            const double zi = (double)(x + dd);
+            z[i] = zi;
+            acc[0] = fma(zi, r[i], acc[0]);
+            if (!first) acc[1] = fma(zi, rp[i], acc[1]);
+        }
+    }
+    __device__ void done(double (&acc)[2]) const { write_partials<2>(acc, part); }
     TT_EPI_CALL
 };
 
@@ -380,7 +412,7 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
     double* yk;
     double* ym;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double&) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const bool clamped = (cl >> c) & 1;
@@ -388,18 +420,18 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
             ym[3 * n + c] = clamped ? 0.0 : yM[c];
         }
     }
-    __device__ void done(double) const {}
+    __device__ void done(double (&)[2]) const {}
     TT_EPI_CALL
 };
 
 struct EpiRaw {  // y = yK + yM, no BC
     double* y;
     template <class S>
-    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, double&) const {
+    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = yK[c] + yM[c];
     }
-    __device__ void done(double) const {}
+    __device__ void done(double (&)[2]) const {}
     TT_EPI_CALL
 };
 
@@ -408,7 +440,7 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     double fs;
     double* R;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double&) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -418,7 +450,7 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
             R[i] = ((cl >> c) & 1) ? 0.0 : r;
         }
     }
-    __device__ void done(double) const {}
+    __device__ void done(double (&)[2]) const {}
     TT_EPI_CALL
 };
 
@@ -428,7 +460,7 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
     double* part;
     template <class S>
     __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st,
-                        double& acc) const {
+                        double (&acc)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -436,11 +468,11 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
             const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
             pnew[i] = p;
             q[i] = v;
-            acc += p * v;
+            acc[0] += p * v;
         }
     }
-    __device__ void done(double acc) const {
-        double a[1] = {acc};
+    __device__ void done(double (&acc)[2]) const {
+        double a[1] = {acc[0]};
         write_partials<1>(a, part);
     }
     TT_EPI_CALL
@@ -580,6 +612,22 @@ void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, con
     }
 }
 
+void launch_cheb_last(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
+                      const void* dinv, double c0, double c1, double* z, const double* r, const double* rp, bool first,
+                      bool fp64, Reducer& red, hipStream_t s) {
+    TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
+    if (fp64) {
+        EpiChebOut<double> e{(const double*)b, (const double*)d, (const double*)dinv, c0, c1, z, r, rp,
+                             first ? 1 : 0, red.part(PB_DOTS)};
+        red.nblk[PB_DOTS] = run_apply<double, false, true>(L, cK, cM, StageOne<double>{(const double*)x}, e, s);
+    } else {
+        EpiChebOut<float> e{(const float*)b, (const float*)d, (const float*)dinv, (float)c0, (float)c1, z, r, rp,
+                            first ? 1 : 0, red.part(PB_DOTS)};
+        red.nblk[PB_DOTS] = run_apply<float, false, true>(L, cK, cM, StageOne<float>{(const float*)x}, e, s);
+    }
+    red.nv[PB_DOTS] = 2;
+}
+
 void launch_apply_split(const OpLevel& L, double cK, double cM, const double* u, double* yk, double* ym,
                         hipStream_t s) {
     run_apply<double, false, true>(L, cK, cM, StageOne<double>{u}, EpiSplit{yk, ym}, s, false);
@@ -688,6 +736,28 @@ __global__ void k_cheb_first(const T* __restrict__ b, const T* __restrict__ dinv
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i < n) x[i] = c1 * dinv[i] * b[i];
 }
+// b = (T) r (FP64 in) and x = c1 D^-1 b: the start of a V-cycle on the finest level
+template <typename T>
+__global__ void k_cheb_first_from64(const double* __restrict__ r, T* __restrict__ b, const T* __restrict__ dinv, T c1,
+                                    T* __restrict__ x, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) {
+        const T bi = (T)r[i];
+        b[i] = bi;
+        x[i] = c1 * dinv[i] * bi;
+    }
+}
+void launch_cheb_first_from64(const double* r, void* b, const void* dinv, double c1, void* x, long n, bool fp64,
+                              hipStream_t s) {
+    if (fp64)
+        hipLaunchKernelGGL(k_cheb_first_from64<double>, dim3(nblocks(n)), dim3(BS), 0, s, r, (double*)b,
+                           (const double*)dinv, c1, (double*)x, n);
+    else
+        hipLaunchKernelGGL(k_cheb_first_from64<float>, dim3(nblocks(n)), dim3(BS), 0, s, r, (float*)b,
+                           (const float*)dinv, (float)c1, (float*)x, n);
+    TT_HIP(hipGetLastError());
+}
+
 void launch_cheb_first(const void* b, const void* dinv, double c1, void* x, long n, bool fp64, hipStream_t s) {
     if (fp64)
         hipLaunchKernelGGL(k_cheb_first<double>, dim3(nblocks(n)), dim3(BS), 0, s, (const double*)b,

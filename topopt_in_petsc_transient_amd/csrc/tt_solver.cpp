@@ -340,7 +340,7 @@ void Multigrid::coarse_solve() {
     if (!lc.fp64) launch_copy(x64, lc.x.p, nc, true, false, s);
 }
 
-void Multigrid::vcycle(int l) {
+void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64, bool first) {
     const int nl = (int)L.size();
     if (l == nl - 1) {
         coarse_solve();
@@ -352,8 +352,10 @@ void Multigrid::vcycle(int l) {
     const long nd = lv.g.nd;
     const double lo = cheb_lo * lv.lmax, hi = cheb_hi * lv.lmax;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-    // pre-smoothing from x = 0
-    launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
+    // pre-smoothing from x = 0 (on the finest level of a Krylov preconditioner call, b = (T) r is
+    // formed by the same kernel)
+    if (r64) launch_cheb_first_from64(r64, lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
+    else launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
     double rho0 = 1.0 / sigma;
     bool d_is_x = true;
     for (int k = 1; k < cheb_deg; k++) {
@@ -381,11 +383,25 @@ void Multigrid::vcycle(int l) {
             c1 = 2.0 * rho1 / delta;
             rho0 = rho1;
         }
+        if (z64 && k == cheb_deg - 1) {  // last step: z = x_new in FP64 plus the flexible-CG dots
+            launch_cheb_last(lv.op, cK, cM, lv.b.p, lv.x.p, k > 0 ? lv.d.p : nullptr, lv.dinv.p, c0, c1, z64, r64,
+                             rp64, first, f64, ctx->red, s);
+            break;
+        }
         launch_cheb_step(lv.op, cK, cM, lv.b.p, lv.x.p, k > 0 ? lv.d.p : nullptr, lv.dinv.p, c0, c1, lv.x2.p,
                          lv.d2.p, f64, s);
         std::swap(lv.x.p, lv.x2.p);
         std::swap(lv.d.p, lv.d2.p);
     }
+}
+
+void Multigrid::apply_fcg(const double* r, const double* rp, double* z, bool first) {
+    if (L.size() == 1) {
+        apply(r, z);
+        launch_fcg_dots(z, r, rp, L[0]->g.nd, ctx->red, first, ctx->stream);
+        return;
+    }
+    vcycle(0, r, z, rp, first);
 }
 
 void Multigrid::apply(const double* r, double* z) {
@@ -446,9 +462,12 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
     double* pnew = kp1.p;
     int it = 0;
     while (!conv && it < prm.maxit) {
-        if (use_mg) mg.apply(kr.p, kz.p);
-        else launch_jacobi(dinvM.p, kr.p, kz.p, nd, stream);
-        launch_fcg_dots(kz.p, kr.p, krp.p, nd, red, first, stream);
+        if (use_mg) {
+            mg.apply_fcg(kr.p, krp.p, kz.p, first);  // z = B r with the dots z.r, z.r_prev
+        } else {
+            launch_jacobi(dinvM.p, kr.p, kz.p, nd, stream);
+            launch_fcg_dots(kz.p, kr.p, krp.p, nd, red, first, stream);
+        }
         launch_fcg_apply(op, cK, cM, kz.p, pold, pnew, kq.p, red, first, stream);
         launch_fcg_update(x, pnew, kr.p, krp.p, kq.p, nd, red, stream);
         std::swap(pold, pnew);
