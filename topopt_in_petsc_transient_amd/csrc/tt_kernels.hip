@@ -297,17 +297,33 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
     __device__ void operator()(bool in, int n, uint8_t cl, const V (&yK)[3], const V (&yM)[3],          \
                                const S& st) const {                                                     \
         double acc[2] = {0.0, 0.0};                                                                    \
-        if (in) put(n, cl, yK, yM, st, acc);                                                           \
+        if (in) {                                                                                      \
+            const V uo[3] = {st.k(3 * n), st.k(3 * n + 1), st.k(3 * n + 2)};                           \
+            put(n, cl, yK, yM, st, uo, acc);                                                           \
+        }                                                                                              \
         done(acc);                                                                                     \
     }
+
+// Input value of dof i for an epilogue: the staged value uo (free dofs), re-read through the stage
+// only on clamped dofs, where the staged copy is masked.
+template <class S, typename T>
+__device__ __forceinline__ T input_at(const S& st, uint8_t cl, int c, int i, const T (&uo)[3]) {
+    T x = uo[c];
+    if ((cl >> c) & 1) x = st.k(i);
+    return x;
+}
 
 template <typename T>
 struct EpiBC {  // y = A u on free dofs, u on clamped
     T* y;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&)[3], double (&)[2]) const {
 #pragma unroll
-        for (int c = 0; c < 3; c++) y[3 * n + c] = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
+        for (int c = 0; c < 3; c++) {
+            T v = yK[c] + yM[c];
+            if ((cl >> c) & 1) v = st.k(3 * n + c);
+            y[3 * n + c] = v;
+        }
     }
     __device__ void done(double (&)[2]) const {}
     TT_EPI_CALL
@@ -318,10 +334,10 @@ struct EpiBCDot {  // y = A u (BC form) and partials of u.y
     T* y;
     double* part;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&acc)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3], double (&acc)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const T u = st.k(3 * n + c);
+            const T u = input_at(st, cl, c, 3 * n + c, uo);
             const T v = ((cl >> c) & 1) ? u : yK[c] + yM[c];
             y[3 * n + c] = v;
             acc[0] += (double)u * (double)v;
@@ -339,10 +355,11 @@ struct EpiResid {  // r = b - A x
     const T* b;
     T* r;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&)[3], double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const T ax = ((cl >> c) & 1) ? st.k(3 * n + c) : yK[c] + yM[c];
+            T ax = yK[c] + yM[c];
+            if ((cl >> c) & 1) ax = st.k(3 * n + c);
             r[3 * n + c] = b[3 * n + c] - ax;
         }
     }
@@ -359,11 +376,12 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
     T* xn;
     T* dn;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
+                        double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
-            const T x = st.k(i);
+            const T x = input_at(st, cl, c, i, uo);
             const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
             const T r = b[i] - ax;
             T dd = c1 * dinv[i] * r;
@@ -390,11 +408,11 @@ struct EpiChebOut {
     int first;
     double* part;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, double (&acc)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3], double (&acc)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
-            const T x = st.k(i);
+            const T x = input_at(st, cl, c, i, uo);
             const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
             T dd = c1 * dinv[i] * (b[i] - ax);
             if (d) dd = fma(c0, d[i], dd);
@@ -412,7 +430,7 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
     double* yk;
     double* ym;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const bool clamped = (cl >> c) & 1;
@@ -427,7 +445,7 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
 struct EpiRaw {  // y = yK + yM, no BC
     double* y;
     template <class S>
-    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, double (&)[2]) const {
+    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = yK[c] + yM[c];
     }
@@ -440,7 +458,7 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     double fs;
     double* R;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -460,11 +478,11 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
     double* part;
     template <class S>
     __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st,
-                        double (&acc)[2]) const {
+                        const double (&uo)[3], double (&acc)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
-            const double p = st.k(i);
+            const double p = input_at(st, cl, c, i, uo);
             const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
             pnew[i] = p;
             q[i] = v;
