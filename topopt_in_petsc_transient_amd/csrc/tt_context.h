@@ -40,7 +40,7 @@ struct Multigrid {
     // coarsest level
     int nc = 0;
     bool dense = true;
-    DBuf<double> Ainv, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest vectors
+    DBuf<double> Ainv, AinvP, sprow, spcol, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest level
     DBuf<int> dinfo;
     void* blas = nullptr;  // rocblas_handle used by the rocSOLVER factorisation of the coarsest operator
     int coarse_deg = 12;

@@ -148,6 +148,11 @@ struct OpLevel {
 };
 
 // MFMA A fragments of the interleaved [KE; M8 (x) I3] matrix, lane order (see tt_kernels.hip)
+// Symmetric coarsest solve on the packed lower triangle (64 x 64 tiles): size of the packed array,
+// packing of a full symmetric matrix, and x = A b with per-tile partials (nt (nt + 1) / 2 x 64 each).
+long sym_packed_size(int n);
+void launch_sym_pack(const double* A, int n, double* P, hipStream_t s);
+void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
 // Column-major lower triangle of an n x n matrix copied into its upper triangle.
 void launch_mirror_lower(double* A, int n, hipStream_t s);
 void mfma_fragments(const double* KE, const double* M8, double* af64, float* af32);
@@ -199,7 +204,6 @@ void launch_convert_coef(const double* src, void* dst, long n, int kind, hipStre
 void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, double pK, int mInt, double Rmin,
                    double Rmax, double pM, const double* P, double* kf, double* mf, double* dkf, double* dmf,
                    double* Pkf, double* Pmf, hipStream_t s);
-void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hipStream_t s);
 
 // dots (FP64): results land in red.scal[slot..]
 void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s);

@@ -951,20 +951,6 @@ void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, 
     TT_HIP(hipGetLastError());
 }
 
-// dense coarsest-level solve x = Ainv b (Ainv symmetric, row-major n x n): one wave per row,
-// lanes stride along the row (coalesced), fixed butterfly sum
-__global__ void k_dense_gemv(const double* __restrict__ A, const double* __restrict__ b, double* __restrict__ x,
-                             int n) {
-    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    const double* Ar = A + (size_t)row * n;
-    double s = 0.0;
-    for (int j = lane; j < n; j += 64) s = fma(Ar[j], b[j], s);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) x[row] = s;
-}
 __global__ void k_mirror_lower(double* A, int n) {
     __shared__ double t[32][33];
     const int bi = blockIdx.x, bj = blockIdx.y;  // tile (bi, bj) of the lower part, bi >= bj
@@ -985,8 +971,94 @@ void launch_mirror_lower(double* A, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_mirror_lower, dim3(nt, nt), dim3(256), 0, s, A, n);
     TT_HIP(hipGetLastError());
 }
-void launch_dense_gemv(const double* Ainv, const double* b, double* x, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4), dim3(256), 0, s, Ainv, b, x, n);
+// ---- symmetric coarsest solve on the packed lower triangle: 64 x 64 tiles (I, J), I >= J, stored
+// contiguously in row-major order, tile index I (I + 1) / 2 + J.  Half the bytes of the full matrix.
+constexpr int ST = 64;
+static inline int sym_tiles(int n) { return (n + ST - 1) / ST; }
+long sym_packed_size(int n) {
+    const long nt = sym_tiles(n);
+    return nt * (nt + 1) / 2 * ST * ST;
+}
+// pack the lower triangle of the full symmetric matrix A (n x n) into tiles (zero padded)
+__global__ void k_sym_pack(const double* __restrict__ A, int n, double* __restrict__ P) {
+    const int I = blockIdx.y, J = blockIdx.x;
+    if (J > I) return;
+    double* T = P + ((size_t)I * (I + 1) / 2 + J) * ST * ST;
+    for (int e = threadIdx.x; e < ST * ST; e += blockDim.x) {
+        const int r = e / ST, c = e % ST, i = I * ST + r, j = J * ST + c;
+        T[e] = (i < n && j < n) ? A[(size_t)i * n + j] : 0.0;
+    }
+}
+void launch_sym_pack(const double* A, int n, double* P, hipStream_t s) {
+    const int nt = sym_tiles(n);
+    hipLaunchKernelGGL(k_sym_pack, dim3(nt, nt), dim3(256), 0, s, A, n, P);
+    TT_HIP(hipGetLastError());
+}
+// per tile: row sums A_IJ x_J and (off the diagonal) column sums A_IJ^T x_I, written as partials.
+// Each thread first loads its 16 tile values (two per 16-byte load, all in flight), then the tile
+// goes through LDS once.
+__global__ __launch_bounds__(256) void k_sym_tiles(const double* __restrict__ P, const double* __restrict__ x, int n,
+                                                   double* __restrict__ prow, double* __restrict__ pcol) {
+    __shared__ double tile[ST][ST + 1];
+    __shared__ double xs[2][ST];
+    __shared__ double red[4][ST];
+    const int id = blockIdx.x;
+    int I = (int)((sqrt(8.0 * id + 1.0) - 1.0) * 0.5);  // I (I + 1) / 2 <= id < (I + 1)(I + 2) / 2
+    while ((I + 1) * (I + 2) / 2 <= id) I++;
+    while (I * (I + 1) / 2 > id) I--;
+    const int J = id - I * (I + 1) / 2;
+    const double2* T2 = reinterpret_cast<const double2*>(P + (size_t)id * ST * ST);
+    const int tid = threadIdx.x;
+    double2 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = T2[tid + 256 * k];
+    if (tid < ST) {
+        const int j = J * ST + tid, i = I * ST + tid;
+        xs[0][tid] = j < n ? x[j] : 0.0;
+        xs[1][tid] = i < n ? x[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int e = 2 * (tid + 256 * k);
+        tile[e / ST][e % ST] = v[k].x;
+        tile[e / ST][e % ST + 1] = v[k].y;
+    }
+    __syncthreads();
+    const int part = tid >> 6, k = tid & 63;  // 4 x 64 threads: rows (parts 0, 1) and columns (parts 2, 3)
+    double sacc = 0.0;
+    if (part < 2) {
+#pragma unroll 8
+        for (int c = part * 32; c < part * 32 + 32; c++) sacc = fma(tile[k][c], xs[0][c], sacc);
+    } else {
+#pragma unroll 8
+        for (int r = (part - 2) * 32; r < (part - 2) * 32 + 32; r++) sacc = fma(tile[r][k], xs[1][r], sacc);
+    }
+    red[part][k] = sacc;
+    __syncthreads();
+    if (tid < ST) prow[(size_t)id * ST + tid] = red[0][tid] + red[1][tid];
+    else if (tid < 2 * ST) pcol[(size_t)id * ST + (tid - ST)] = I == J ? 0.0 : red[2][tid - ST] + red[3][tid - ST];
+}
+// y_i = sum_{J <= I} prow(I, J) + sum_{K > I} pcol(K, I): one block of 256 threads per tile row I,
+// 4 threads per output summing every 4th partial, combined in a fixed order
+__global__ __launch_bounds__(256) void k_sym_sum(const double* __restrict__ prow, const double* __restrict__ pcol,
+                                                 int n, int nt, double* __restrict__ y) {
+    __shared__ double red[4][ST];
+    const int I = blockIdx.x, q = threadIdx.x >> 6, r = threadIdx.x & 63;
+    double s = 0.0;
+    for (int m = q; m < nt; m += 4) {  // m <= I: tile (I, m) rows; m > I: tile (m, I) columns
+        s += m <= I ? prow[((size_t)I * (I + 1) / 2 + m) * ST + r] : pcol[((size_t)m * (m + 1) / 2 + I) * ST + r];
+    }
+    red[q][r] = s;
+    __syncthreads();
+    const int i = I * ST + threadIdx.x;
+    if (threadIdx.x < ST && i < n) y[i] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol,
+                      hipStream_t s) {
+    const int nt = sym_tiles(n);
+    hipLaunchKernelGGL(k_sym_tiles, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, P, b, n, prow, pcol);
+    TT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sym_sum, dim3(nt), dim3(256), 0, s, prow, pcol, n, nt, x);
     TT_HIP(hipGetLastError());
 }
 

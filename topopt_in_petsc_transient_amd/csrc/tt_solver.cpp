@@ -114,6 +114,12 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
     cx.alloc(nc);
     if (dense) {
         Ainv.alloc((size_t)nc * nc);
+        AinvP.alloc(sym_packed_size(nc));
+        {
+            const long nt = (nc + 63) / 64, ntiles = nt * (nt + 1) / 2;
+            sprow.alloc(ntiles * 64);
+            spcol.alloc(ntiles * 64);
+        }
         dinfo.alloc(1);
         rocblas_handle h = nullptr;
         TT_CHECK(rocblas_create_handle(&h) == rocblas_status_success, TT_ERR_HIP, "rocblas_create_handle failed");
@@ -252,6 +258,7 @@ void Multigrid::assemble_coarse_inverse() {
     TT_HIP(hipStreamSynchronize(s));
     TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
     launch_mirror_lower(Ainv.p, n, s);
+    launch_sym_pack(Ainv.p, n, AinvP.p, s);
     TT_HIP(hipStreamSynchronize(s));
 }
 
@@ -316,7 +323,7 @@ void Multigrid::coarse_solve() {
         x64 = cx.p;
     }
     if (dense) {
-        launch_dense_gemv(Ainv.p, b64, x64, nc, s);
+        launch_sym_solve(AinvP.p, b64, x64, nc, sprow.p, spcol.p, s);
     } else {
         OpLevel op = coarse_op64();
         const double theta = 0.5 * (clmax + clmin), delta = 0.5 * (clmax - clmin), sigma = theta / delta;
