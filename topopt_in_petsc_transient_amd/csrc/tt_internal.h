@@ -190,6 +190,10 @@ void launch_cheb_first(const void* b, const void* dinv, double c1, void* x, long
 // b = (T) r and x = c1 D^-1 b (finest level, r in FP64)
 void launch_cheb_first_from64(const double* r, void* b, const void* dinv, double c1, void* x, long n, bool fp64,
                               hipStream_t s);
+// first two Chebyshev iterates in one operator apply: x0 = c1_first D^-1 b formed on the fly
+// (b = (T) r64 written out when r64 is given), then dn = c0 x0 + c1 D^-1 (b - A x0), xn = x0 + dn
+void launch_cheb_first_step(const OpLevel& L, double cK, double cM, const double* r64, void* b, const void* dinv,
+                            double c1_first, double c0, double c1, void* xn, void* dn, bool fp64, hipStream_t s);
 // last post-smoothing step of the finest level: z (FP64) = x + dn, partials of z.r and z.r_prev into PB_DOTS
 void launch_cheb_last(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
                       const void* dinv, double c0, double c1, double* z, const double* r, const double* rp, bool first,

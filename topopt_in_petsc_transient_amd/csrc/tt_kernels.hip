@@ -199,6 +199,18 @@ struct StageFcg {
     __device__ __forceinline__ T k(int i) const { return first ? z[i] : z[i] + beta * pold[i]; }
     __device__ __forceinline__ T m(int i) const { return k(i); }
 };
+// First Chebyshev iterate x0 = c1 D^-1 b formed on the fly, with b = (T) r (FP64) on the finest level
+template <typename T, bool FROM64>
+struct StageChebFirst {
+    const double* r64;
+    const T* b;
+    const T* dinv;
+    T c1;
+    __device__ __forceinline__ void prologue() {}
+    __device__ __forceinline__ T bval(int i) const { return FROM64 ? (T)r64[i] : b[i]; }
+    __device__ __forceinline__ T k(int i) const { return c1 * dinv[i] * bval(i); }
+    __device__ __forceinline__ T m(int i) const { return k(i); }
+};
 struct StageThree {  // wK = sum_i cK_i in_i, wM = sum_i cM_i in_i (FP64)
     const double *u, *v, *a;
     double k0, k1, k2, m0, m1, m2;
@@ -388,6 +400,34 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
             if (d) dd = fma(c0, d[i], dd);
             xn[i] = x + dd;
             if (dn) dn[i] = dd;
+        }
+    }
+    __device__ void done(double (&)[2]) const {}
+    TT_EPI_CALL
+};
+
+// Second Chebyshev step from x0 = c1 D^-1 b (staged by StageChebFirst, so d = x0):
+// dn = c0 x0 + c1 D^-1 (b - A x0), xn = x0 + dn; on the finest level it also writes b = (T) r.
+template <typename T, bool FROM64>
+struct EpiChebFirst {
+    T* bout;
+    const T* dinv;
+    T c0, c1;
+    T* xn;
+    T* dn;
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
+                        double (&)[2]) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            const T bi = st.bval(i);
+            if (FROM64) bout[i] = bi;
+            const T x = input_at(st, cl, c, i, uo);
+            const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
+            const T dd = fma(c0, x, c1 * dinv[i] * (bi - ax));
+            xn[i] = x + dd;
+            dn[i] = dd;
         }
     }
     __device__ void done(double (&)[2]) const {}
@@ -627,6 +667,33 @@ void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, con
         EpiCheb<float> e{(const float*)b,  (const float*)d, (const float*)dinv, (float)c0,
                          (float)c1,        (float*)xn,      (float*)dn};
         run_apply<float, false, true>(L, cK, cM, StageOne<float>{(const float*)x}, e, s);
+    }
+}
+
+void launch_cheb_first_step(const OpLevel& L, double cK, double cM, const double* r64, void* b, const void* dinv,
+                            double c1_first, double c0, double c1, void* xn, void* dn, bool fp64, hipStream_t s) {
+    if (fp64) {
+        const double* dv = (const double*)dinv;
+        if (r64)
+            run_apply<double, false, true>(L, cK, cM, StageChebFirst<double, true>{r64, nullptr, dv, c1_first},
+                                           EpiChebFirst<double, true>{(double*)b, dv, c0, c1, (double*)xn,
+                                                                      (double*)dn}, s);
+        else
+            run_apply<double, false, true>(L, cK, cM,
+                                           StageChebFirst<double, false>{nullptr, (const double*)b, dv, c1_first},
+                                           EpiChebFirst<double, false>{nullptr, dv, c0, c1, (double*)xn,
+                                                                       (double*)dn}, s);
+    } else {
+        const float* dv = (const float*)dinv;
+        if (r64)
+            run_apply<float, false, true>(L, cK, cM, StageChebFirst<float, true>{r64, nullptr, dv, (float)c1_first},
+                                          EpiChebFirst<float, true>{(float*)b, dv, (float)c0, (float)c1,
+                                                                    (float*)xn, (float*)dn}, s);
+        else
+            run_apply<float, false, true>(L, cK, cM,
+                                          StageChebFirst<float, false>{nullptr, (const float*)b, dv, (float)c1_first},
+                                          EpiChebFirst<float, false>{nullptr, dv, (float)c0, (float)c1, (float*)xn,
+                                                                     (float*)dn}, s);
     }
 }
 

@@ -361,11 +361,25 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     // pre-smoothing from x = 0 (on the finest level of a Krylov preconditioner call, b = (T) r is
     // formed by the same kernel)
-    if (r64) launch_cheb_first_from64(r64, lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
-    else launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
     double rho0 = 1.0 / sigma;
     bool d_is_x = true;
-    for (int k = 1; k < cheb_deg; k++) {
+    int k0 = 1;
+    if (cheb_deg >= 2) {
+        // x0 = D^-1 b / theta is formed inside the staging of the first step's operator apply
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        launch_cheb_first_step(lv.op, cK, cM, r64, lv.b.p, lv.dinv.p, 1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta,
+                               lv.x2.p, lv.d2.p, f64, s);
+        std::swap(lv.x.p, lv.x2.p);
+        std::swap(lv.d.p, lv.d2.p);
+        d_is_x = false;
+        rho0 = rho1;
+        k0 = 2;
+    } else if (r64) {
+        launch_cheb_first_from64(r64, lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
+    } else {
+        launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
+    }
+    for (int k = k0; k < cheb_deg; k++) {
         const double rho1 = 1.0 / (2.0 * sigma - rho0);
         launch_cheb_step(lv.op, cK, cM, lv.b.p, lv.x.p, d_is_x ? lv.x.p : lv.d.p, lv.dinv.p, rho1 * rho0,
                          2.0 * rho1 / delta, lv.x2.p, lv.d2.p, f64, s);
