@@ -130,6 +130,9 @@ struct Context {
     SolveResult solve_A(const double* b, double* x);         // Newmark operator, MG-preconditioned FCG
     SolveResult solve_M(const double* b, double* x);         // mass operator N M N + I - N, Jacobi FCG
     SolveResult fcg(double cK, double cM, bool use_mg, const double* b, double* x);
+    int fcg_hint[2] = {0, 0};  // iterations of the last multigrid / Jacobi preconditioned solve
+    // sums of two single-value partial regions with one host round trip
+    void reduce2(int r1, int r2, double& a, double& b);
     void sync();
     double read_scal(int slot);
     double reduce1(int region);  // host sum of a partial region (synchronises)

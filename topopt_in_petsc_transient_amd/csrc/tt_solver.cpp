@@ -437,6 +437,17 @@ void Multigrid::apply(const double* r, double* z) {
 // ------------------------------------------------------------------ context-level Krylov
 void Context::sync() { TT_HIP(hipStreamSynchronize(stream)); }
 
+void Context::reduce2(int r1, int r2, double& a, double& b) {
+    const int n1 = red.nblk[r1] * red.nv[r1], n2 = red.nblk[r2] * red.nv[r2];
+    TT_HIP(hipMemcpyAsync(red.hpart, red.part(r1), sizeof(double) * n1, hipMemcpyDeviceToHost, stream));
+    TT_HIP(hipMemcpyAsync(red.hpart + n1, red.part(r2), sizeof(double) * n2, hipMemcpyDeviceToHost, stream));
+    TT_HIP(hipStreamSynchronize(stream));
+    a = 0.0;
+    b = 0.0;
+    for (int i = 0; i < n1; i++) a += red.hpart[i];
+    for (int i = 0; i < n2; i++) b += red.hpart[n1 + i];
+}
+
 double Context::read_scal(int slot) {
     TT_HIP(hipMemcpyAsync(red.host + slot, red.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, stream));
     TT_HIP(hipStreamSynchronize(stream));
@@ -466,8 +477,12 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
     launch_residual(op, cK, cM, b, x, kr.p, true, stream);
     launch_dot(b, b, nd, red, PB_MISC, stream);
     launch_dot(kr.p, kr.p, nd, red, PB_MISC2, stream);
-    const double bn = std::sqrt(reduce1(PB_MISC));
-    const double rr0 = reduce1(PB_MISC2);
+    double bb = 0.0, rr0 = 0.0;
+    reduce2(PB_MISC, PB_MISC2, bb, rr0);
+    const double bn = std::sqrt(bb);
+    // The previous solve with this preconditioner took `hint` iterations: the residual is read back
+    // (a host round trip) only from iteration hint - 2 on.  Converging earlier only adds iterations.
+    int& hint = fcg_hint[use_mg ? 0 : 1];
     SolveResult res;
     stats.solves++;
     if (!std::isfinite(bn)) throw Error{TT_ERR_NONFINITE, "non-finite right-hand side"};
@@ -494,6 +509,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
         std::swap(pold, pnew);
         first = false;
         it++;
+        if (it < hint - 2) continue;
         rel = std::sqrt(reduce1(PB_RR)) / bn;
         if (!std::isfinite(rel)) throw Error{TT_ERR_NONFINITE, "non-finite residual in flexible CG"};
         if (rel <= rtol) {
@@ -505,6 +521,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
             else first = true;
         }
     }
+    hint = it;
     stats.krylov_iters += it;
     stats.last_iters = it;
     stats.last_relres = rel;
