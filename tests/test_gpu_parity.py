@@ -187,3 +187,54 @@ def test_nonfinite_density_is_an_error():
     with pytest.raises(capi.TTError) as e:
         c.set_density(x)
     assert e.value.code == capi.TT_ERR_NONFINITE
+
+
+# The reference's other presets (LinearElasticity.cc:222-335 regions, :337-591 boundaries, :593-675
+# loads, :185-219 time integration, :1580-1623 interpolations) on a 12 x 4 x 4 box of 12 x 4 x 8
+# elements, over a time window that contains the Gaussian-cosine signal's peak (t0 = 8).
+@pytest.mark.parametrize("boundaries,loadloc,loadsignal,ti,sInt,mInt,region,objective", [
+    (0, 0, 0, 1, 1, 0, 0, 3),
+    (1, 1, 0, 1, 1, 0, 1, 3),
+    (1, 2, 1, 0, 2, 1, 3, 3),
+    (0, 1, 1, 1, 1, 1, 2, 6),
+])
+def test_reference_presets_fp64(boundaries, loadloc, loadsignal, ti, sInt, mInt, region, objective):
+    kw = dict(boundaries=boundaries, loadloc=loadloc, loadsignal=loadsignal, ti=ti, sInt=sInt, mInt=mInt,
+              region=region)
+    po, pg = problem(nelx=12, nely=4, nelz=8, nsteps=24, dt=0.5, pulse_steps=6, objective=objective, nlvls=2,
+                     lx=12.0, ly=4.0, lz=4.0, **kw)
+    if sInt == 2:
+        po.penalK = pg.penalK = 4.0
+    x = random_density(O.nel(po), seed=17)
+    assert np.array_equal(O.bc_mask(po) > 0, _gpu_free_mask(pg))
+    states = O.forward(po, x)
+    fx_ref, df_ref = O.objective_gradient(po, x, states)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    for t in (1, 12, 24):
+        u, v, a = c.get_state(t)
+        assert rel(u, states[t, 0]) < FP64_TOL and rel(a, states[t, 2]) < FP64_TOL, t
+    fx, df = c.objective_gradient()
+    assert fx_ref != 0.0
+    assert abs(fx - fx_ref) / abs(fx_ref) < FP64_TOL
+    assert rel(df, df_ref) < FP64_TOL
+
+
+def _gpu_free_mask(pg):
+    """Free dofs seen by the GPU path: K applied to the unit vector field is the identity on clamped dofs."""
+    c = capi.Context(pg)
+    n = pg.nelx * pg.nely * pg.nelz
+    c.set_density(np.full(n, 0.5))
+    nd = c.nd
+    e = np.zeros(nd)
+    y = np.zeros(nd)
+    free = np.ones(nd, dtype=bool)
+    for comp in range(3):  # probe one component at a time: clamped rows return the input unchanged
+        e[:] = 0.0
+        e[comp::3] = 1e-3
+        c.op_apply(capi.TT_OP_M, e, y, prec=0)
+        clamped = np.isclose(y[comp::3], 1e-3, rtol=0, atol=1e-15)
+        free[comp::3] = ~clamped
+    c.close()
+    return free
