@@ -35,7 +35,7 @@ PEAK_FP64 = 78.6e12        # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_FP32 = 157.3e12       # MI355X dense FP32 matrix, flop/s
 # HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
 # scaled by the counters' reading for a copy kernel of known size); None until measured.
-TRAFFIC_A32 = 20225724  # FETCH 6679.6 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (copy-calibrated)
+TRAFFIC_A32 = 20227778  # FETCH 6680.6 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (copy-calibrated)
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
@@ -215,8 +215,8 @@ def main():
     ach = fl / (a32["fp32_ms"] * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": ach / (PEAK_FP32 / 1e12), "traffic": TRAFFIC_A32,
-            "kernel": "k_apply_mfma<float,float,4,...,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
-                      "128x64x64)",
+            "kernel": "k_apply_tiles<float,float,4,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
+                      "128x64x64, 8x8x4-element tiles on MFMA 16x16x4)",
             "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl}
     value = args.steps / mixed["seconds"]
     rec = {
