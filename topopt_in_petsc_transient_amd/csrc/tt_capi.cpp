@@ -59,10 +59,10 @@ Context::Context(const tt_params& p) : prm(p) {
     TT_HIP(hipMemcpy(M832d.p, m832, sizeof(m832), hipMemcpyHostToDevice));
     {
         double af64[18 * 64];
-        float af32[18 * 64];
-        mfma_fragments(KE, M8, af64, af32);
+        float af32[AF32_WORDS];
+        mfma_fragments(KE, M8, af64, af32, reinterpret_cast<uint32_t*>(af32 + 18 * 64), afh_scale);
         AF64d.alloc(18 * 64);
-        AF32d.alloc(18 * 64);
+        AF32d.alloc(AF32_WORDS);
         TT_HIP(hipMemcpy(AF64d.p, af64, sizeof(af64), hipMemcpyHostToDevice));
         TT_HIP(hipMemcpy(AF32d.p, af32, sizeof(af32), hipMemcpyHostToDevice));
     }
@@ -132,6 +132,9 @@ Context::Context(const tt_params& p) : prm(p) {
     op_fine64.M832 = M832d.p;
     op_fine64.AF64 = AF64d.p;
     op_fine64.AF32 = AF32d.p;
+    op_fine64.AFH = reinterpret_cast<const uint32_t*>(AF32d.p + 18 * 64);
+    op_fine64.hs[0] = afh_scale[0];
+    op_fine64.hs[1] = afh_scale[1];
     op_fine64.mask = mask.p;
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);

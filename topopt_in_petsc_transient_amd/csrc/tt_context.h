@@ -95,6 +95,7 @@ struct Context {
     OpLevel op_fine64{};  // FP64 fine operator view (for FCG residuals)
     DBuf<double> KE64d, M864d, AF64d;
     DBuf<float> KE32d, M832d, AF32d;
+    float afh_scale[2];
     DBuf<float> kf32, mf32;  // FP32 fine coefficients for the FP32 microbenchmark apply
     bool density_set = false;
     bool initial_done = false;

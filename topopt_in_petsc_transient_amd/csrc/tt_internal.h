@@ -145,6 +145,8 @@ struct OpLevel {
     const uint8_t* mask;   // per node clamped bits
     const double* AF64;    // MFMA A fragments of [KE; M8 (x) I3] in lane order (18 x 64)
     const float* AF32;
+    const uint32_t* AFH;   // FP16-split fragments (3 x 2 x 64 x 4 words), K / M rows scaled by 1 / hs
+    float hs[2];
 };
 
 // MFMA A fragments of the interleaved [KE; M8 (x) I3] matrix, lane order (see tt_kernels.hip)
@@ -155,7 +157,8 @@ void launch_sym_pack(const double* A, int n, double* P, hipStream_t s);
 void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
 // Column-major lower triangle of an n x n matrix copied into its upper triangle.
 void launch_mirror_lower(double* A, int n, hipStream_t s);
-void mfma_fragments(const double* KE, const double* M8, double* af64, float* af32);
+constexpr int AF32_WORDS = 18 * 64 + 3 * 2 * 64 * 4;  // FP32 fragments followed by the FP16-split ones
+void mfma_fragments(const double* KE, const double* M8, double* af64, float* af32, uint32_t* afh, float* hs);
 
 // y = N.(cK*K + cM*M)(N.u) on free dofs, identity on clamped dofs.  With red: partials of u.y in
 // partial region dot_slot (PB_*).  Reductions below likewise take a partial region.

@@ -8,6 +8,10 @@
 // and the element coefficients.  No scatter, no atomics: results are
 // bit-reproducible.  Epilogue functors fuse the Krylov / Chebyshev vector work
 // into the same pass.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "tt_internal.h"
 
 namespace tt {
@@ -592,7 +596,13 @@ static int run_apply(const OpLevel& L, double cK, double cM, const Stage& st, co
     const T* M8 = std::is_same<T, double>::value ? (const T*)L.M864 : (const T*)L.M832;
     GridArgs ga = gargs(L.g);
     if (!TWO && allow_mfma && apply_impl() == 1) {
-        const T* AF = std::is_same<T, double>::value ? (const T*)L.AF64 : (const T*)L.AF32;
+        // FP32 runs on the FP16-split fragments, whose K / M rows carry the scales hs
+        const bool f32 = std::is_same<T, float>::value;
+        const T* AF = f32 ? (const T*)L.AFH : (const T*)L.AF64;
+        if (f32) {
+            cK *= L.hs[0];
+            cM *= L.hs[1];
+        }
         // 8x8x4 tiles on grids with enough of them to fill the chip, 8x8x2 on the coarse levels
         const int tx = (L.g.nx + 7) / 8, ty = (L.g.ny + 7) / 8;
         const bool deep = (long)tx * ty * ((L.g.nz + 3) / 4) >= 4L * device_cus();

@@ -76,10 +76,10 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
         TT_HIP(hipMemcpy(lev->M832.p, m832, sizeof(m832), hipMemcpyHostToDevice));
         {
             double af64[18 * 64];
-            float af32[18 * 64];
-            mfma_fragments(ke, m8, af64, af32);
+            float af32[AF32_WORDS];
+            mfma_fragments(ke, m8, af64, af32, reinterpret_cast<uint32_t*>(af32 + 18 * 64), lev->op.hs);
             lev->AF64.alloc(18 * 64);
-            lev->AF32.alloc(18 * 64);
+            lev->AF32.alloc(AF32_WORDS);
             TT_HIP(hipMemcpy(lev->AF64.p, af64, sizeof(af64), hipMemcpyHostToDevice));
             TT_HIP(hipMemcpy(lev->AF32.p, af32, sizeof(af32), hipMemcpyHostToDevice));
         }
@@ -105,6 +105,7 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
         lev->op.M832 = lev->M832.p;
         lev->op.AF64 = lev->AF64.p;
         lev->op.AF32 = lev->AF32.p;
+        lev->op.AFH = reinterpret_cast<const uint32_t*>(lev->AF32.p + 18 * 64);
         L.push_back(std::move(lev));
     }
     const Grid& gc = L.back()->g;
