@@ -125,6 +125,7 @@ enum {
     SC_TMP1,
     SC_TMP2,
     SC_TMP3,
+    SC_DONE,        // flexible CG: 1 once the recursive residual met the tolerance (x is then frozen)
     SC_FX0,         // objective accumulators start here
     SC_COUNT = 32
 };
@@ -176,9 +177,11 @@ void launch_apply_split(const OpLevel& L, double cK, double cM, const double* u,
 // unmasked K/M apply with arbitrary coefficient arrays (FP64): y = cK*K(kf)u + cM*M(mf)u
 void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, double cK, double cM, const double* u,
                       double* y, hipStream_t s);
-// Newmark right-hand side: R = N.( fs*F + sum_i (K-weights, M-weights) applied to u,v,a )
+// Newmark right-hand side: R = N.( fs*F + sum_i (K-weights, M-weights) applied to u,v,a ), and the
+// solve's initial guess G = u + g1 v + g2 a
 void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, const double* a, const double* F,
-                        double fs, const double wK[3], const double wM[3], double* R, hipStream_t s);
+                        double fs, const double wK[3], const double wM[3], double* R, double* G, double g1,
+                        double g2, hipStream_t s);
 // diagonal of (cK K + cM M) with identity on clamped dofs, inverted
 void launch_diag_inv(const OpLevel& L, double cK, double cM, void* dinv, bool fp64, hipStream_t s);
 
@@ -216,8 +219,11 @@ void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, 
 void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s);
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s);
+// x += alpha p, r -= alpha q with alpha = gamma / p.q, unless the previous iteration's recursive
+// residual already met r.r <= tol2 (then alpha = 0 from there on, so the result does not depend on
+// when the host looks at the residual); `first` (a (re)start) clears that state.
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
-                       hipStream_t s);
+                       double tol2, bool first, hipStream_t s);
 void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipStream_t s);
 // power iteration: x = dinv.Ax with partials of ||x||^2 in PB_MISC; then x /= ||x||
 void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s);

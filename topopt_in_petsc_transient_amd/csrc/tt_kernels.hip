@@ -497,12 +497,14 @@ struct EpiRaw {  // y = yK + yM, no BC
     TT_EPI_CALL
 };
 
-struct EpiRHS {  // R = N.(fs F + yK + yM)
+struct EpiRHS {  // R = N.(fs F + yK + yM); G = u + g1 v + g2 a (the solve's initial guess, if G)
     const double* F;
     double fs;
     double* R;
+    double* G;
+    double g1, g2;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st, const double (&)[3], double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -510,6 +512,7 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
             r += yK[c];
             r += yM[c];
             R[i] = ((cl >> c) & 1) ? 0.0 : r;
+            if (G) G[i] = fma(g2, st.a[i], fma(g1, st.v[i], st.u[i]));
         }
     }
     __device__ void done(double (&)[2]) const {}
@@ -738,9 +741,10 @@ void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, doub
 }
 
 void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, const double* a, const double* F,
-                        double fs, const double wK[3], const double wM[3], double* R, hipStream_t s) {
+                        double fs, const double wK[3], const double wM[3], double* R, double* G, double g1,
+                        double g2, hipStream_t s) {
     StageThree st{u, v, a, wK[0], wK[1], wK[2], wM[0], wM[1], wM[2]};
-    run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R}, s);
+    run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R, G, g1, g2}, s);
 }
 
 __global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first);
@@ -1191,12 +1195,18 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
 }
 
 // alpha = gamma / (p.q) from the partials of the FCG apply (one block); gamma becomes gamma_old.
-__global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, double* scal) {
-    double pq[1];
+// Not on a (re)start: once the previous update's r.r (its partials) is <= tol2, SC_DONE is set
+// and alpha is 0 for the rest of the solve.
+__global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, const double* __restrict__ rr_part,
+                            int rr_nblk, double* scal, double tol2, int first) {
+    double pq[1], rr[1] = {0.0};
     sum_partials<1>(pq_part, pq_nblk, pq);
+    if (!first) sum_partials<1>(rr_part, rr_nblk, rr);
     if (threadIdx.x == 0) {
+        const bool done = !first && (scal[SC_DONE] != 0.0 || rr[0] <= tol2);
         const double gam = scal[SC_GAMMA];
-        scal[SC_ALPHA] = gam / pq[0];
+        scal[SC_DONE] = done ? 1.0 : 0.0;
+        scal[SC_ALPHA] = done ? 0.0 : gam / pq[0];
         scal[SC_GAMMA_OLD] = gam;
     }
 }
@@ -1218,8 +1228,9 @@ __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ 
     write_partials<1>(acc, part);
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(BS), 0, s, red.part(PB_PQ), red.nblk[PB_PQ], red.scal.p);
+                       double tol2, bool first, hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(BS), 0, s, red.part(PB_PQ), red.nblk[PB_PQ], red.part(PB_RR),
+                       red.nblk[PB_RR], red.scal.p, tol2, first ? 1 : 0);
     TT_HIP(hipGetLastError());
     const int nb = rb_blocks(n);
     hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.scal.p, red.part(PB_RR));
