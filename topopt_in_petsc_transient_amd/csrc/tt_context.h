@@ -113,7 +113,7 @@ struct Context {
     DBuf<double> st_cur, st_next;
 
     // adjoint work
-    DBuf<double> lam, lamn, lamT, mu, nu, mun, nun, hdu, hddu, hdddu, Kl, Ml, rhs, df, fxs, nu0;
+    DBuf<double> lam, lamn, lamn2, lamn3, lamT, mu, nu, mun, nun, hdu, hddu, hdddu, Kl, Ml, rhs, df, fxs, nu0;
 
     // PDE filter
     double KF[64];

@@ -293,5 +293,8 @@ void launch_fill_scal(void* x, long n, double v, hipStream_t s);
 void launch_scale(double* x, long n, double a, hipStream_t s);
 // out = a*x + b*y (out may alias x or y)
 void launch_lincomb(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
+// out = a x + b y + c z
+void launch_lincomb3(double a, const double* x, double b, const double* y, double c, const double* z, double* out,
+                     long n, hipStream_t s);
 
 }  // namespace tt

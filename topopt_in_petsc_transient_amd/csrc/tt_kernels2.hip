@@ -570,5 +570,15 @@ void launch_lincomb(double a, const double* x, double b, const double* y, double
     hipLaunchKernelGGL(k_lincomb, dim3(nblk(n)), dim3(BS2), 0, s, a, x, b, y, out, n);
     TT_HIP(hipGetLastError());
 }
+__global__ void k_lincomb3(double a, const double* x, double b, const double* y, double c, const double* z,
+                           double* out, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = fma(c, z[i], fma(b, y[i], a * x[i]));
+}
+void launch_lincomb3(double a, const double* x, double b, const double* y, double c, const double* z, double* out,
+                     long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_lincomb3, dim3(nblk(n)), dim3(BS2), 0, s, a, x, b, y, c, z, out, n);
+    TT_HIP(hipGetLastError());
+}
 
 }  // namespace tt

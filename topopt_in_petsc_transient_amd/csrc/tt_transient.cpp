@@ -271,6 +271,7 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     };
 
     // one backward step at t: needs state t and state t-1
+    int nhist = 0;  // multipliers of later steps held in lamn, lamn2, lamn3
     auto adjoint_step = [&](int t, const double* st, const double* stb) {
         grad_state(st);
         if (t == T) {
@@ -281,11 +282,20 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
             launch_apply_split(op_fine64, 1.0, 1.0, lamn.p, Kl.p, Ml.p, stream);
             launch_adjoint_combine(Kl.p, Ml.p, hdu.p, hddu.p, hdddu.p, mun.p, nun.p, mask.p, coef, false, mu.p, nu.p,
                                    rhs.p, nn, stream);
-            launch_copy(lamn.p, lam.p, nd, true, true, stream);  // initial guess lambda_{t+1} (:917)
+            // Initial guess: quadratic extrapolation of lambda_{t+1}, lambda_{t+2}, lambda_{t+3} (linear,
+            // then constant, near the end).  The reference starts from lambda_{t+1} (:917); the converged
+            // multiplier is the same to the solver tolerance.
+            if (nhist >= 3) launch_lincomb3(3.0, lamn.p, -3.0, lamn2.p, 1.0, lamn3.p, lam.p, nd, stream);
+            else if (nhist == 2) launch_lincomb(2.0, lamn.p, -1.0, lamn2.p, lam.p, nd, stream);
+            else launch_copy(lamn.p, lam.p, nd, true, true, stream);
         }
         solve_A(rhs.p, lam.p);
         sens(0, lam.p, st, stb, t);
-        std::swap(lam.p, lamn.p);
+        // lambda_t becomes lambda_{t+1} of the next (earlier) step
+        std::swap(lamn3.p, lamn2.p);
+        std::swap(lamn2.p, lamn.p);
+        std::swap(lamn.p, lam.p);
+        nhist = std::min(nhist + 1, 3);
         std::swap(mu.p, mun.p);
         std::swap(nu.p, nun.p);
         stats.adjoint_steps++;
