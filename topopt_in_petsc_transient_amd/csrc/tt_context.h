@@ -63,7 +63,7 @@ struct Multigrid {
    private:
     void vcycle(int l, const double* r64 = nullptr, double* z64 = nullptr, const double* rp64 = nullptr,
                 bool first = false);
-    void coarse_solve();
+    void coarse_solve(bool direct64);
     double power_lmax(int l, int iters);
     void assemble_coarse_inverse();
     OpLevel coarse_op64() const;

@@ -177,11 +177,9 @@ void launch_apply_split(const OpLevel& L, double cK, double cM, const double* u,
 // unmasked K/M apply with arbitrary coefficient arrays (FP64): y = cK*K(kf)u + cM*M(mf)u
 void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, double cK, double cM, const double* u,
                       double* y, hipStream_t s);
-// Newmark right-hand side: R = N.( fs*F + sum_i (K-weights, M-weights) applied to u,v,a ), and the
-// solve's initial guess G = u + g1 v + g2 a
+// Newmark right-hand side: R = N.( fs*F + sum_i (K-weights, M-weights) applied to u,v,a )
 void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, const double* a, const double* F,
-                        double fs, const double wK[3], const double wM[3], double* R, double* G, double g1,
-                        double g2, hipStream_t s);
+                        double fs, const double wK[3], const double wM[3], double* R, hipStream_t s);
 // diagonal of (cK K + cM M) with identity on clamped dofs, inverted
 void launch_diag_inv(const OpLevel& L, double cK, double cM, void* dinv, bool fp64, hipStream_t s);
 
@@ -204,10 +202,12 @@ void launch_cheb_first_step(const OpLevel& L, double cK, double cM, const double
 void launch_cheb_last(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
                       const void* dinv, double c0, double c1, double* z, const double* r, const double* rp, bool first,
                       bool fp64, Reducer& red, hipStream_t s);
+// bc = R rf (fine precision fp64; an FP32 fine level may write an FP64 coarse vector, out64)
 void launch_restrict(const Grid& fine, const Grid& coarse, const void* rf, const uint8_t* cmask, void* bc,
-                     bool fp64, hipStream_t s);
+                     bool fp64, bool out64, hipStream_t s);
+// xf += P xc (fine precision fp64; an FP32 fine level may read an FP64 coarse vector, in64)
 void launch_prolong_add(const Grid& fine, const Grid& coarse, const void* xc, const uint8_t* fmask, void* xf,
-                        bool fp64, hipStream_t s);
+                        bool fp64, bool in64, hipStream_t s);
 void launch_coarsen_coef(const Grid& fine, const Grid& coarse, const double* kf_f, const double* mf_f, double* kf_c,
                          double* mf_c, hipStream_t s);
 void launch_convert_coef(const double* src, void* dst, long n, int kind, hipStream_t s);

@@ -497,14 +497,12 @@ struct EpiRaw {  // y = yK + yM, no BC
     TT_EPI_CALL
 };
 
-struct EpiRHS {  // R = N.(fs F + yK + yM); G = u + g1 v + g2 a (the solve's initial guess, if G)
+struct EpiRHS {  // R = N.(fs F + yK + yM)
     const double* F;
     double fs;
     double* R;
-    double* G;
-    double g1, g2;
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st, const double (&)[3], double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -512,7 +510,6 @@ struct EpiRHS {  // R = N.(fs F + yK + yM); G = u + g1 v + g2 a (the solve's ini
             r += yK[c];
             r += yM[c];
             R[i] = ((cl >> c) & 1) ? 0.0 : r;
-            if (G) G[i] = fma(g2, st.a[i], fma(g1, st.v[i], st.u[i]));
         }
     }
     __device__ void done(double (&)[2]) const {}
@@ -741,10 +738,9 @@ void launch_apply_raw(const OpLevel& L, const double* kf, const double* mf, doub
 }
 
 void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, const double* a, const double* F,
-                        double fs, const double wK[3], const double wM[3], double* R, double* G, double g1,
-                        double g2, hipStream_t s) {
+                        double fs, const double wK[3], const double wM[3], double* R, hipStream_t s) {
     StageThree st{u, v, a, wK[0], wK[1], wK[2], wM[0], wM[1], wM[2]};
-    run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R, G, g1, g2}, s);
+    run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R}, s);
 }
 
 __global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first);
@@ -867,27 +863,36 @@ void launch_cheb_first(const void* b, const void* dinv, double c1, void* x, long
     TT_HIP(hipGetLastError());
 }
 
-// restriction R = P^T of trilinear interpolation (PETSc DMDA Q1 interpolation, LinearElasticity.cc:2840-2844)
-template <typename T>
-__global__ void k_restrict(GridArgs f, GridArgs c, const T* __restrict__ rf, const uint8_t* __restrict__ cmask,
-                           T* __restrict__ bc) {
+// restriction R = P^T of trilinear interpolation (PETSc DMDA Q1 interpolation, LinearElasticity.cc:2840-2844).
+// Branch-free and fully unrolled so that the 27 neighbour loads are in flight together: neighbours
+// outside the fine grid get weight 0 at a clamped (valid) index, which leaves the sum unchanged.
+// The coarse output may be wider than the fine input (FP32 levels feeding an FP64 coarsest solve).
+template <typename Tf, typename Tc>
+__global__ void k_restrict(GridArgs f, GridArgs c, const Tf* __restrict__ rf, const uint8_t* __restrict__ cmask,
+                           Tc* __restrict__ bc) {
     const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (n >= (long)c.nx * c.ny * c.nz) return;
     const int I = n % c.nx, J = (n / c.nx) % c.ny, K = n / ((long)c.nx * c.ny);
-    T s0 = 0, s1 = 0, s2 = 0;
+    Tf s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
     for (int dk = -1; dk <= 1; dk++) {
         const int k = 2 * K + dk;
-        if (k < 0 || k >= f.nz) continue;
-        const T wk = dk == 0 ? (T)1 : (T)0.5;
+        const bool kin = k >= 0 && k < f.nz;
+        const Tf wk = kin ? (dk == 0 ? (Tf)1 : (Tf)0.5) : (Tf)0;
+        const int kc = kin ? k : 2 * K;
+#pragma unroll
         for (int dj = -1; dj <= 1; dj++) {
             const int j = 2 * J + dj;
-            if (j < 0 || j >= f.ny) continue;
-            const T wj = dj == 0 ? wk : wk * (T)0.5;
+            const bool jin = j >= 0 && j < f.ny;
+            const Tf wj = jin ? (dj == 0 ? wk : wk * (Tf)0.5) : (Tf)0;
+            const int jc = jin ? j : 2 * J;
+#pragma unroll
             for (int di = -1; di <= 1; di++) {
                 const int i = 2 * I + di;
-                if (i < 0 || i >= f.nx) continue;
-                const T w = di == 0 ? wj : wj * (T)0.5;
-                const long m = i + (long)f.nx * (j + (long)f.ny * k);
+                const bool iin = i >= 0 && i < f.nx;
+                const Tf w = iin ? (di == 0 ? wj : wj * (Tf)0.5) : (Tf)0;
+                const int ic = iin ? i : 2 * I;
+                const long m = ic + (long)f.nx * (jc + (long)f.ny * kc);
                 s0 = fma(w, rf[3 * m], s0);
                 s1 = fma(w, rf[3 * m + 1], s1);
                 s2 = fma(w, rf[3 * m + 2], s2);
@@ -895,46 +900,53 @@ __global__ void k_restrict(GridArgs f, GridArgs c, const T* __restrict__ rf, con
         }
     }
     const uint8_t cl = cmask[n];
-    bc[3 * n] = (cl & 1) ? (T)0 : s0;
-    bc[3 * n + 1] = (cl & 2) ? (T)0 : s1;
-    bc[3 * n + 2] = (cl & 4) ? (T)0 : s2;
+    bc[3 * n] = (cl & 1) ? (Tc)0 : (Tc)s0;
+    bc[3 * n + 1] = (cl & 2) ? (Tc)0 : (Tc)s1;
+    bc[3 * n + 2] = (cl & 4) ? (Tc)0 : (Tc)s2;
 }
 void launch_restrict(const Grid& fine, const Grid& coarse, const void* rf, const uint8_t* cmask, void* bc, bool fp64,
-                     hipStream_t s) {
+                     bool out64, hipStream_t s) {
+    const dim3 gr(nblocks(coarse.nn)), bl(BS);
     if (fp64)
-        hipLaunchKernelGGL(k_restrict<double>, dim3(nblocks(coarse.nn)), dim3(BS), 0, s, gargs(fine), gargs(coarse),
-                           (const double*)rf, cmask, (double*)bc);
+        hipLaunchKernelGGL((k_restrict<double, double>), gr, bl, 0, s, gargs(fine), gargs(coarse), (const double*)rf,
+                           cmask, (double*)bc);
+    else if (out64)
+        hipLaunchKernelGGL((k_restrict<float, double>), gr, bl, 0, s, gargs(fine), gargs(coarse), (const float*)rf,
+                           cmask, (double*)bc);
     else
-        hipLaunchKernelGGL(k_restrict<float>, dim3(nblocks(coarse.nn)), dim3(BS), 0, s, gargs(fine), gargs(coarse),
-                           (const float*)rf, cmask, (float*)bc);
+        hipLaunchKernelGGL((k_restrict<float, float>), gr, bl, 0, s, gargs(fine), gargs(coarse), (const float*)rf,
+                           cmask, (float*)bc);
     TT_HIP(hipGetLastError());
 }
 
-template <typename T>
-__global__ void k_prolong_add(GridArgs f, GridArgs c, const T* __restrict__ xc, const uint8_t* __restrict__ fmask,
-                              T* __restrict__ xf) {
+// xf += P xc on free dofs.  Each fine node interpolates from 1, 2, 4 or 8 coarse nodes; the
+// unrolled 2 x 2 x 2 loop gives a second node of a direction weight 0 (at a valid index) where the
+// fine index is even, so the sum is the same as over the contributing nodes only.
+template <typename Tf, typename Tc>
+__global__ void k_prolong_add(GridArgs f, GridArgs c, const Tc* __restrict__ xc, const uint8_t* __restrict__ fmask,
+                              Tf* __restrict__ xf) {
     const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (n >= (long)f.nx * f.ny * f.nz) return;
     const int i = n % f.nx, j = (n / f.nx) % f.ny, k = n / ((long)f.nx * f.ny);
     const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
-    const int I1 = I0 + (i & 1), J1 = J0 + (j & 1), K1 = K0 + (k & 1);
-    T s0 = 0, s1 = 0, s2 = 0;
+    const int oi = i & 1, oj = j & 1, ok = k & 1;
+    Tf s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
     for (int kk = 0; kk < 2; kk++) {
-        if (kk == 1 && K1 == K0) break;
-        const int K = kk ? K1 : K0;
-        const T wk = (K1 == K0) ? (T)1 : (T)0.5;
+        const Tf wk = ok ? (Tf)0.5 : (kk ? (Tf)0 : (Tf)1);
+        const int K = K0 + (kk & ok);
+#pragma unroll
         for (int jj = 0; jj < 2; jj++) {
-            if (jj == 1 && J1 == J0) break;
-            const int J = jj ? J1 : J0;
-            const T wj = (J1 == J0) ? wk : wk * (T)0.5;
+            const Tf wj = oj ? wk * (Tf)0.5 : (jj ? (Tf)0 : wk);
+            const int J = J0 + (jj & oj);
+#pragma unroll
             for (int ii = 0; ii < 2; ii++) {
-                if (ii == 1 && I1 == I0) break;
-                const int I = ii ? I1 : I0;
-                const T w = (I1 == I0) ? wj : wj * (T)0.5;
+                const Tf w = oi ? wj * (Tf)0.5 : (ii ? (Tf)0 : wj);
+                const int I = I0 + (ii & oi);
                 const long m = I + (long)c.nx * (J + (long)c.ny * K);
-                s0 = fma(w, xc[3 * m], s0);
-                s1 = fma(w, xc[3 * m + 1], s1);
-                s2 = fma(w, xc[3 * m + 2], s2);
+                s0 = fma(w, (Tf)xc[3 * m], s0);
+                s1 = fma(w, (Tf)xc[3 * m + 1], s1);
+                s2 = fma(w, (Tf)xc[3 * m + 2], s2);
             }
         }
     }
@@ -944,12 +956,16 @@ __global__ void k_prolong_add(GridArgs f, GridArgs c, const T* __restrict__ xc, 
     if (!(cl & 4)) xf[3 * n + 2] += s2;
 }
 void launch_prolong_add(const Grid& fine, const Grid& coarse, const void* xc, const uint8_t* fmask, void* xf,
-                        bool fp64, hipStream_t s) {
+                        bool fp64, bool in64, hipStream_t s) {
+    const dim3 gr(nblocks(fine.nn)), bl(BS);
     if (fp64)
-        hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblocks(fine.nn)), dim3(BS), 0, s, gargs(fine),
-                           gargs(coarse), (const double*)xc, fmask, (double*)xf);
+        hipLaunchKernelGGL((k_prolong_add<double, double>), gr, bl, 0, s, gargs(fine), gargs(coarse),
+                           (const double*)xc, fmask, (double*)xf);
+    else if (in64)
+        hipLaunchKernelGGL((k_prolong_add<float, double>), gr, bl, 0, s, gargs(fine), gargs(coarse),
+                           (const double*)xc, fmask, (float*)xf);
     else
-        hipLaunchKernelGGL(k_prolong_add<float>, dim3(nblocks(fine.nn)), dim3(BS), 0, s, gargs(fine), gargs(coarse),
+        hipLaunchKernelGGL((k_prolong_add<float, float>), gr, bl, 0, s, gargs(fine), gargs(coarse),
                            (const float*)xc, fmask, (float*)xf);
     TT_HIP(hipGetLastError());
 }

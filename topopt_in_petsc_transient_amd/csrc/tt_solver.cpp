@@ -313,15 +313,20 @@ OpLevel Multigrid::coarse_op64() const {
     return op;
 }
 
-void Multigrid::coarse_solve() {
+// direct64: the FP32 level above restricted straight into cb and prolongs straight from cx.
+void Multigrid::coarse_solve(bool direct64) {
     MGLevel& lc = *L.back();
     hipStream_t s = ctx->stream;
     const double* b64 = (const double*)lc.b.p;
     double* x64 = (double*)lc.x.p;
     if (!lc.fp64) {
-        launch_copy(lc.b.p, cb.p, nc, false, true, s);
         b64 = cb.p;
         x64 = cx.p;
+        if (direct64) {
+            launch_sym_solve(AinvP.p, b64, x64, nc, sprow.p, spcol.p, s);
+            return;
+        }
+        launch_copy(lc.b.p, cb.p, nc, false, true, s);
     }
     if (dense) {
         launch_sym_solve(AinvP.p, b64, x64, nc, sprow.p, spcol.p, s);
@@ -351,7 +356,7 @@ void Multigrid::coarse_solve() {
 void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64, bool first) {
     const int nl = (int)L.size();
     if (l == nl - 1) {
-        coarse_solve();
+        coarse_solve(l > 0 && dense && !L[l]->fp64);  // as decided by the level above (direct64)
         return;
     }
     MGLevel& lv = *L[l];
@@ -392,9 +397,12 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
     // coarse-grid correction
     launch_residual(lv.op, cK, cM, lv.b.p, lv.x.p, lv.r.p, f64, s);
     MGLevel& lc = *L[l + 1];
-    launch_restrict(lv.g, lc.g, lv.r.p, lc.mask.p, lc.b.p, f64, s);
+    // an FP32 level above the dense FP64 coarsest solve restricts into and prolongs from its FP64
+    // vectors directly
+    const bool direct64 = l + 2 == nl && dense && !lc.fp64;
+    launch_restrict(lv.g, lc.g, lv.r.p, lc.mask.p, direct64 ? (void*)cb.p : lc.b.p, f64, direct64, s);
     vcycle(l + 1);
-    launch_prolong_add(lv.g, lc.g, lc.x.p, lv.mask.p, lv.x.p, f64, s);
+    launch_prolong_add(lv.g, lc.g, direct64 ? (const void*)cx.p : lc.x.p, lv.mask.p, lv.x.p, f64, direct64, s);
     // post-smoothing
     rho0 = 1.0 / sigma;
     for (int k = 0; k < cheb_deg; k++) {

@@ -46,8 +46,7 @@ void Context::complete_initial_state() {
     launch_fill(s0, 0.0, 3 * nd, true, stream);  // u0 = v0 = 0 and the solve's initial guess a0 = 0
     const double wK[3] = {-1.0, -prm.dbeta, 0.0};
     const double wM[3] = {0.0, -prm.dalpha, 0.0};
-    launch_newmark_rhs(op_fine64, s0, s0 + nd, s0 + 2 * nd, F.p, load_intensity(0), wK, wM, rhs.p, nullptr, 0.0, 0.0,
-                       stream);
+    launch_newmark_rhs(op_fine64, s0, s0 + nd, s0 + 2 * nd, F.p, load_intensity(0), wK, wM, rhs.p, stream);
     solve_M(rhs.p, s0 + 2 * nd);
     for (auto& v : slot_step) v = -1;
     slot_step[0] = 0;
@@ -66,12 +65,12 @@ void Context::newmark_step(const double* s_in, double* s_out, int t) {
     const double al = prm.dalpha, be = prm.dbeta;
     const double wK[3] = {a[2] * be, -a[0] * be, -a[1] * be};
     const double wM[3] = {a[5] + a[2] * al, a[3] - a[0] * al, a[4] - a[1] * al};
-    // Initial guess: the displacement for which the corrector returns a_{n+1} = a_n,
+    // Initial guess (second kernel): the displacement for which the corrector returns a_{n+1} = a_n,
     // u_n + (a3 v_n + (1 + a4) a_n) / a5 (u_n + dt v_n + dt^2/2 a_n for Newmark, + dt^2 a_n for
     // backward Euler).  The reference starts from u_n (:1358); the converged answer is the same
     // to the solver tolerance, and the guess saves Krylov iterations on smooth motion.
-    launch_newmark_rhs(op_fine64, u, v, ac, F.p, load_intensity(t), wK, wM, rhs.p, un, a[3] / a[5],
-                       (1.0 + a[4]) / a[5], stream);
+    launch_newmark_rhs(op_fine64, u, v, ac, F.p, load_intensity(t), wK, wM, rhs.p, stream);
+    launch_lincomb3(1.0, u, a[3] / a[5], v, (1.0 + a[4]) / a[5], ac, un, nd, stream);
     solve_A(rhs.p, un);
     launch_newmark_update(u, v, ac, un, vn, an, a, nd, stream);
     stats.forward_steps++;
