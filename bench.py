@@ -39,13 +39,15 @@ TRAFFIC_A32 = 20227778  # FETCH 6680.6 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (c
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
+# Chebyshev smoother degree per precision mode (each mode at its fastest setting on this workload)
+CHEB = {0: 2, 1: 2}
 
 
 def params(capi, precision, nsteps, slots=0, nlvls=None):
     return capi.default_params(
         nelx=NEL[0], nely=NEL[1], nelz=NEL[2], xc=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0), nsteps=nsteps, t1=DT * nsteps,
         pulse_steps=20, precision=precision, rtol=1e-8, maxit=200, nlvls=nlvls or NLVLS,
-        cheb_degree=2, checkpoint_slots=slots, rmin=0.08, filter_rtol=1e-10)
+        cheb_degree=CHEB.get(precision, 2), checkpoint_slots=slots, rmin=0.08, filter_rtol=1e-10)
 
 
 def density(capi):
@@ -163,9 +165,15 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--ops-only", action="store_true", help="only the operator microbenchmark (for profiling)")
     ap.add_argument("--nlvls", type=int, default=4, help="multigrid levels (4: coarsest grid 16x8x8 elements)")
+    ap.add_argument("--cheb-mixed", type=int, default=None, help="Chebyshev degree of the mixed-precision multigrid")
+    ap.add_argument("--cheb-fp64", type=int, default=None, help="Chebyshev degree of the FP64 multigrid")
     args = ap.parse_args()
     global NLVLS
     NLVLS = args.nlvls
+    if args.cheb_mixed:
+        CHEB[1] = args.cheb_mixed
+    if args.cheb_fp64:
+        CHEB[0] = args.cheb_fp64
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU domain decomposition is a design note only).", file=sys.stderr)
@@ -225,14 +233,15 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64 (mixed: FP32/BF16 multigrid, FP64 outer CG)",
         "data": "synthetic (seeded filtered-noise density, half-sine line load)",
         "config": {"workload": "128x64x64 hex8 cantilever, Newmark dt=0.005, 4-level... see nlvls",
-                   "nlvls": NLVLS, "rtol": 1e-8, "cheb_degree": 2,
+                   "nlvls": NLVLS, "rtol": 1e-8, "cheb_degree_mixed": CHEB[1], "cheb_degree_fp64": CHEB[0],
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": args.steps / fp64["seconds"], "mixed_vs_fp64_speedup": fp64["seconds"] / mixed["seconds"],
         "u_final_rel_err_mixed_vs_fp64": u_err, "forward_mixed": mixed, "forward_fp64": fp64,
         "adjoint_gradient": grad, "operator": ops, "roofline": roof, "cpu_baseline": cpu,
     }
     rec["config"]["workload"] = (f"128x64x64 hex8 cantilever (1,635,075 dofs), Newmark beta=1/4 dt={DT}, "
-                                 f"{rec['config']['nlvls']}-level MG Chebyshev(2) in FP64 FCG rtol 1e-8")
+                                 f"{rec['config']['nlvls']}-level MG, Chebyshev({CHEB[1]}) mixed / ({CHEB[0]}) FP64, "
+                                 f"in FP64 FCG rtol 1e-8")
     print(json.dumps(rec))
 
 

@@ -57,6 +57,31 @@ def test_forward_states_fp64():
                 assert rel(arr, states[t, q]) < FP64_TOL, (t, q)
 
 
+@pytest.mark.parametrize("precision", [capi.TT_PREC_FP64, capi.TT_PREC_MIXED])
+def test_newmark_solves_meet_rtol_on_the_true_residual(precision):
+    """At rtol >= 1e-9 the solver stops on the recursive residual without recomputing b - A x; the true
+    residual of every step (the equation-of-motion residual M a + C v + K u - f, which equals
+    A u_t - b_t for the Newmark corrector) must still meet rtol ||b_t||."""
+    rtol = 1e-8
+    po, pg = problem(nsteps=20, rtol=rtol, precision=precision)
+    x = random_density(O.nel(po), seed=12)
+    c = _ctx(pg, x)
+    c.complete_initial_state()
+    c.forward()
+    K = O.assemble(po, x, 1.0, 0.0)
+    M = O.assemble(po, x, 0.0, 1.0)
+    C = po.dalpha * M + po.dbeta * K
+    a = O.newmark_consts(1, po.t1 / po.nsteps)
+    A = (1.0 + a[2] * po.dbeta) * K + (a[5] + a[2] * po.dalpha) * M
+    N = O.bc_mask(po)
+    F = O.load_location(po)
+    for t in range(1, po.nsteps + 1):
+        u, v, acc = c.get_state(t)
+        r = (M @ acc + C @ v + K @ u - F * O.load_intensity(po, t)) * N
+        b = (A @ u) * N - r
+        assert np.linalg.norm(r) <= rtol * np.linalg.norm(b) * (1.0 + 1e-6), t
+
+
 def test_objective_gradient_fp64():
     po, pg = problem()
     x = random_density(O.nel(po), seed=4)

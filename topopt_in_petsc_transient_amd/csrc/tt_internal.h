@@ -126,6 +126,8 @@ enum {
     SC_TMP2,
     SC_TMP3,
     SC_DONE,        // flexible CG: 1 once the recursive residual met the tolerance (x is then frozen)
+    SC_TOL2,        // flexible CG: (rtol ||b||)^2
+    SC_ZERO,        // flexible CG: 1 if b = 0 (the solution is x = 0)
     SC_FX0,         // objective accumulators start here
     SC_COUNT = 32
 };
@@ -219,11 +221,14 @@ void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, 
 void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s);
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s);
-// x += alpha p, r -= alpha q with alpha = gamma / p.q, unless the previous iteration's recursive
-// residual already met r.r <= tol2 (then alpha = 0 from there on, so the result does not depend on
-// when the host looks at the residual); `first` (a (re)start) clears that state.
+// Flexible-CG (re)start from the b.b and r.r partial regions: device tolerance (rtol ||b||)^2, the
+// b = 0 flag, and whether the current x already meets the tolerance.
+void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, hipStream_t s);
+// x += alpha p, r -= alpha q with alpha = gamma / p.q, unless x already met the tolerance (set at the
+// (re)start, or by the previous update's r.r after its first iteration): then alpha = 0 from there on,
+// so the result does not depend on when the host looks at the residual.  x = 0 if b = 0.
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
-                       double tol2, bool first, hipStream_t s);
+                       bool first, hipStream_t s);
 void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipStream_t s);
 // power iteration: x = dinv.Ax with partials of ||x||^2 in PB_MISC; then x /= ||x||
 void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s);

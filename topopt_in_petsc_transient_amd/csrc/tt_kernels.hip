@@ -76,12 +76,25 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* part) {
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nblk, double (&out)[NV]) {
     __shared__ double sh[16 * NV];
-    double acc[NV];
+    // four independent accumulators per thread so that four loads are in flight; fixed order
+    double acc[NV], a4[3][NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+    for (int k = 0; k < NV; k++) acc[k] = a4[0][k] = a4[1][k] = a4[2][k] = 0.0;
+    const int st = blockDim.x;
+    int b = threadIdx.x;
+    for (; b + 3 * st < nblk; b += 4 * st)
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            acc[k] += part[(size_t)b * NV + k];
+            a4[0][k] += part[(size_t)(b + st) * NV + k];
+            a4[1][k] += part[(size_t)(b + 2 * st) * NV + k];
+            a4[2][k] += part[(size_t)(b + 3 * st) * NV + k];
+        }
+    for (; b < nblk; b += st)
 #pragma unroll
         for (int k = 0; k < NV; k++) acc[k] += part[(size_t)b * NV + k];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] += (a4[0][k] + a4[1][k]) + a4[2][k];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x + 63) >> 6;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -1192,13 +1205,14 @@ __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restric
     }
     write_partials<2>(acc, part);
 }
-// gamma = z.r and beta = (z.r - z.r_prev) / gamma_old (0 on a (re)start) from the partials of k_fcg_dots
+// gamma = z.r and beta = (z.r - z.r_prev) / gamma_old (0 on a (re)start, and once the solve is
+// frozen, when gamma_old may be 0) from the partials of k_fcg_dots
 __global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first) {
     double d[2];
     sum_partials<2>(dots_part, dots_nblk, d);
     if (threadIdx.x == 0) {
         scal[SC_GAMMA] = d[0];
-        scal[SC_BETA] = first ? 0.0 : (d[0] - d[1]) / scal[SC_GAMMA_OLD];
+        scal[SC_BETA] = (first || scal[SC_DONE] != 0.0) ? 0.0 : (d[0] - d[1]) / scal[SC_GAMMA_OLD];
     }
 }
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
@@ -1210,16 +1224,37 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
     TT_HIP(hipGetLastError());
 }
 
+// Start (or restart) of a flexible-CG solve, from the partials of b.b and of r.r for the current
+// x: the tolerance (rtol ||b||)^2, whether b = 0, and whether x already meets the tolerance.
+__global__ void k_fcg_init(const double* __restrict__ bb_part, int bb_nblk, const double* __restrict__ rr_part,
+                           int rr_nblk, double rtol2, double* scal) {
+    double bb[1], rr[1];
+    sum_partials<1>(bb_part, bb_nblk, bb);
+    sum_partials<1>(rr_part, rr_nblk, rr);
+    if (threadIdx.x == 0) {
+        const double tol2 = rtol2 * bb[0];
+        scal[SC_TOL2] = tol2;
+        scal[SC_ZERO] = bb[0] == 0.0 ? 1.0 : 0.0;
+        scal[SC_DONE] = (bb[0] == 0.0 || rr[0] <= tol2) ? 1.0 : 0.0;
+    }
+}
+void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(BS), 0, s, red.part(bb_region), red.nblk[bb_region],
+                       red.part(rr_region), red.nblk[rr_region], rtol2, red.scal.p);
+    TT_HIP(hipGetLastError());
+}
+
 // alpha = gamma / (p.q) from the partials of the FCG apply (one block); gamma becomes gamma_old.
-// Not on a (re)start: once the previous update's r.r (its partials) is <= tol2, SC_DONE is set
-// and alpha is 0 for the rest of the solve.
+// After the first iteration of a (re)start, the previous update's r.r (its partials) is checked
+// too: once it is <= (rtol ||b||)^2, SC_DONE stays set and alpha is 0 for the rest of the solve,
+// so x is frozen at the first iteration that met the tolerance.
 __global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, const double* __restrict__ rr_part,
-                            int rr_nblk, double* scal, double tol2, int first) {
+                            int rr_nblk, double* scal, int first) {
     double pq[1], rr[1] = {0.0};
     sum_partials<1>(pq_part, pq_nblk, pq);
     if (!first) sum_partials<1>(rr_part, rr_nblk, rr);
     if (threadIdx.x == 0) {
-        const bool done = !first && (scal[SC_DONE] != 0.0 || rr[0] <= tol2);
+        const bool done = scal[SC_DONE] != 0.0 || (!first && rr[0] <= scal[SC_TOL2]);
         const double gam = scal[SC_GAMMA];
         scal[SC_DONE] = done ? 1.0 : 0.0;
         scal[SC_ALPHA] = done ? 0.0 : gam / pq[0];
@@ -1227,26 +1262,29 @@ __global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, con
     }
 }
 
-// x += alpha p; r_prev = r; r -= alpha q; partials of r.r.
+// x += alpha p (x = 0 if b = 0); r_prev = r; r -= alpha q; partials of r.r.  A frozen solve keeps
+// x and r as they are.
 __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ p, double* __restrict__ r,
                              double* __restrict__ rp, const double* __restrict__ q, long n,
                              const double* __restrict__ scal, double* part) {
     const double al = scal[SC_ALPHA];
+    const bool zero = scal[SC_ZERO] != 0.0, done = scal[SC_DONE] != 0.0;
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        x[i] = fma(al, p[i], x[i]);
+        if (zero) x[i] = 0.0;
+        else if (!done) x[i] = fma(al, p[i], x[i]);
         const double ri = r[i];
         rp[i] = ri;
-        const double rn = fma(-al, q[i], ri);
+        const double rn = done ? ri : fma(-al, q[i], ri);
         r[i] = rn;
         acc[0] = fma(rn, rn, acc[0]);
     }
     write_partials<1>(acc, part);
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
-                       double tol2, bool first, hipStream_t s) {
+                       bool first, hipStream_t s) {
     hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(BS), 0, s, red.part(PB_PQ), red.nblk[PB_PQ], red.part(PB_RR),
-                       red.nblk[PB_RR], red.scal.p, tol2, first ? 1 : 0);
+                       red.nblk[PB_RR], red.scal.p, first ? 1 : 0);
     TT_HIP(hipGetLastError());
     const int nb = rb_blocks(n);
     hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.scal.p, red.part(PB_RR));
