@@ -66,13 +66,17 @@ typedef struct tt_params {
     double rmin;                   /* PDE filter radius (-rmin); R = rmin/(2 sqrt 3) */
     double filter_rtol;            /* PDE filter solve tolerance */
     int verbose;
+    int graphs;                    /* 1: each multigrid-preconditioned Krylov iteration is replayed as a
+                                      captured HIP graph (bit-identical results; default 0: measured
+                                      slower than eager launches on ROCm 7 for this ~30-kernel
+                                      iteration) */
 } tt_params;
 
 typedef struct tt_stats {
     long solves;          /* linear solves since the last reset */
     long krylov_iters;    /* flexible-CG iterations since the last reset */
     int last_iters;
-    double last_relres;   /* true relative residual of the last solve */
+    double last_relres;   /* relative residual of the last solve (recursive; the recomputed b - A x when rtol < 1e-9) */
     double max_relres;
     long forward_steps;   /* Newmark steps integrated (incl. recomputation) */
     long adjoint_steps;

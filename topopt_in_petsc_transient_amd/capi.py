@@ -43,6 +43,7 @@ class Params(ctypes.Structure):
         ("nlvls", ctypes.c_int), ("precision", ctypes.c_int), ("rtol", ctypes.c_double), ("maxit", ctypes.c_int),
         ("cheb_degree", ctypes.c_int), ("checkpoint_slots", ctypes.c_int),
         ("rmin", ctypes.c_double), ("filter_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
+        ("graphs", ctypes.c_int),
     ]
 
 

@@ -139,7 +139,7 @@ Context::Context(const tt_params& p) : prm(p) {
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
 
-    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &dinvM, &lam, &lamn, &lamn2, &lamn3, &lamT, &mu, &nu, &mun, &nun, &hdu,
+    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &dinvM, &kx, &lam, &lamn, &lamn2, &lamn3, &lamT, &mu, &nu, &mun, &nun, &hdu,
                             &hddu, &hdddu, &Kl, &Ml, &rhs, &nu0})
         b->alloc(nd);
     TT_HIP(hipMemsetAsync(lamT.p, 0, nd * 8, stream));
@@ -179,7 +179,14 @@ void Context::set_filter_radius(double rmin) {
 }
 
 Context::~Context() {
+    clear_graphs();
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::clear_graphs() {
+    for (auto& kv : iter_graphs)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    iter_graphs.clear();
 }
 
 // ------------------------------------------------------------------ PDE filter solve (Jacobi PCG, FP64)
@@ -388,6 +395,7 @@ int tt_default_params(tt_params* p) {
     p->checkpoint_slots = 0;
     p->rmin = 0.08; p->filter_rtol = 1e-10;
     p->verbose = 0;
+    p->graphs = 0;
     return TT_OK;
 }
 

@@ -1,6 +1,7 @@
 // Library context: device data of one transient-elasticity problem on one GPU.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -52,6 +53,8 @@ struct Multigrid {
     ~Multigrid();
 
     void build(Context* c, int nlvls, bool mixed_prec, int deg);
+    // the level work buffers a V-cycle rotates (x, x2, d, d2 of every smoothing level)
+    std::vector<char**> rotating();
     void set_coefficients(const double* kf_fine, const double* mf_fine);
     void setup_operator(double cK_, double cM_);
     // z = B r (FP64 in and out)
@@ -103,8 +106,19 @@ struct Context {
 
     Multigrid mg;
 
-    // Krylov work vectors (FP64, nd)
-    DBuf<double> kr, krp, kz, kp0, kp1, kq, dinvM;
+    // Krylov work vectors (FP64, nd); kx holds the solution of a graph-replayed solve
+    DBuf<double> kr, krp, kz, kp0, kp1, kq, dinvM, kx;
+
+    // One multigrid-preconditioned flexible-CG iteration captured as a HIP graph, keyed by everything
+    // its launches bake in (pointers of the rotating buffers, first-iteration flag, operator scalars),
+    // with the rotation state and reducer bookkeeping the iteration leaves behind.
+    struct IterGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<char*> after;
+        int nblk[PB_COUNT], nv[PB_COUNT];
+    };
+    std::map<std::vector<uintptr_t>, IterGraph> iter_graphs;
+    void clear_graphs();  // (the multigrid set-up changes baked-in Chebyshev scalars)
 
     // time states: slot storage (u, v, a contiguous, 3*nd each)
     std::vector<DBuf<double>> slots;

@@ -263,7 +263,15 @@ void Multigrid::assemble_coarse_inverse() {
     TT_HIP(hipStreamSynchronize(s));
 }
 
+std::vector<char**> Multigrid::rotating() {
+    std::vector<char**> v;
+    for (size_t l = 0; l + 1 < L.size(); l++)
+        for (DBuf<char>* b : {&L[l]->x, &L[l]->x2, &L[l]->d, &L[l]->d2}) v.push_back(&b->p);
+    return v;
+}
+
 void Multigrid::setup_operator(double cK_, double cM_) {
+    ctx->clear_graphs();
     cK = cK_;
     cM = cM_;
     hipStream_t s = ctx->stream;
@@ -503,8 +511,12 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
     bool first = true, conv = false;
     double* pold = kp0.p;
     double* pnew = kp1.p;
-    int it = 0;
-    while (!conv && it < prm.maxit) {
+    // A multigrid-preconditioned iteration is replayed as a HIP graph; its solution vector is then
+    // the fixed kx (copied in and out) so that one graph serves every solve.
+    const bool graphed = use_mg && prm.graphs;
+    double* const xs = graphed ? kx.p : x;
+    if (graphed) launch_copy(x, xs, nd, true, true, stream);
+    auto iteration = [&]() {
         if (use_mg) {
             mg.apply_fcg(kr.p, krp.p, kz.p, first);  // z = B r with the dots z.r, z.r_prev
         } else {
@@ -512,7 +524,49 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
             launch_fcg_dots(kz.p, kr.p, krp.p, nd, red, first, stream);
         }
         launch_fcg_apply(op, cK, cM, kz.p, pold, pnew, kq.p, red, first, stream);
-        launch_fcg_update(x, pnew, kr.p, krp.p, kq.p, nd, red, first, stream);
+        launch_fcg_update(xs, pnew, kr.p, krp.p, kq.p, nd, red, first, stream);
+    };
+    int it = 0;
+    while (!conv && it < prm.maxit) {
+        if (graphed) {
+            const std::vector<char**> rot = mg.rotating();
+            std::vector<uintptr_t> key = {(uintptr_t)first, (uintptr_t)pold, (uintptr_t)pnew};
+            uint64_t cb[2];
+            std::memcpy(cb, &cK, 8);
+            std::memcpy(cb + 1, &cM, 8);
+            key.push_back((uintptr_t)cb[0]);
+            key.push_back((uintptr_t)cb[1]);
+            for (char** q : rot) key.push_back((uintptr_t)*q);
+            auto gi = iter_graphs.find(key);
+            if (gi == iter_graphs.end()) {
+                TT_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    iteration();
+                } catch (...) {
+                    hipGraph_t dropped = nullptr;
+                    (void)hipStreamEndCapture(stream, &dropped);
+                    if (dropped) (void)hipGraphDestroy(dropped);
+                    throw;
+                }
+                hipGraph_t gr = nullptr;
+                TT_HIP(hipStreamEndCapture(stream, &gr));
+                IterGraph ig;
+                const hipError_t e = hipGraphInstantiate(&ig.exec, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                TT_HIP(e);
+                for (char** q : rot) ig.after.push_back(*q);  // rotation done by the captured launches
+                std::memcpy(ig.nblk, red.nblk, sizeof(ig.nblk));
+                std::memcpy(ig.nv, red.nv, sizeof(ig.nv));
+                gi = iter_graphs.emplace(std::move(key), ig).first;
+            } else {
+                for (size_t k = 0; k < rot.size(); k++) *rot[k] = gi->second.after[k];
+                std::memcpy(red.nblk, gi->second.nblk, sizeof(red.nblk));
+                std::memcpy(red.nv, gi->second.nv, sizeof(red.nv));
+            }
+            TT_HIP(hipGraphLaunch(gi->second.exec, stream));
+        } else {
+            iteration();
+        }
         std::swap(pold, pnew);
         first = false;
         it++;
@@ -536,7 +590,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
                 break;
             }
             // confirm with the true residual; restart the recursion if it drifted
-            launch_residual(op, cK, cM, b, x, kr.p, true, stream);
+            launch_residual(op, cK, cM, b, xs, kr.p, true, stream);
             launch_dot(kr.p, kr.p, nd, red, PB_MISC2, stream);
             rel = std::sqrt(reduce1(PB_MISC2) / bb);
             if (rel <= rtol) {
@@ -547,6 +601,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
             }
         }
     }
+    if (graphed) launch_copy(xs, x, nd, true, true, stream);
     hint = it;
     stats.krylov_iters += it;
     stats.last_iters = it;
