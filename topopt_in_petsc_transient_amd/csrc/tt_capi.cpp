@@ -20,6 +20,10 @@ static void check_params(const tt_params& p) {
     TT_CHECK(p.loadsignal != 1 || p.pulse_steps > 0, TT_ERR_ARG, "pulse_steps must be positive");
     long nn = (long)(p.nelx + 1) * (p.nely + 1) * (p.nelz + 1);
     TT_CHECK(3 * nn < (1L << 31), TT_ERR_ARG, "too many dofs for 32-bit indexing");
+    // the kernels address FP64 dof vectors through 32-bit byte offsets and form node indices with
+    // 24-bit multiplies over one y-z cross-section
+    TT_CHECK(3 * nn * 8 < (1L << 32), TT_ERR_ARG, "too many dofs for 32-bit byte offsets (nodes must be < 178,956,970)");
+    TT_CHECK((long)(p.nely + 1) * (p.nelz + 1) < (1L << 24), TT_ERR_ARG, "y-z cross-section above 2^24 nodes");
 }
 
 Context::Context(const tt_params& p) : prm(p) {
@@ -94,6 +98,15 @@ Context::Context(const tt_params& p) : prm(p) {
             }
     mask.alloc(nn);
     TT_HIP(hipMemcpy(mask.p, m.data(), nn, hipMemcpyHostToDevice));
+    {
+        std::vector<uint8_t> f;
+        tile_bc_flags(g, m, 4, f);
+        tbc4.alloc(f.size());
+        TT_HIP(hipMemcpy(tbc4.p, f.data(), f.size(), hipMemcpyHostToDevice));
+        tile_bc_flags(g, m, 2, f);
+        tbc2.alloc(f.size());
+        TT_HIP(hipMemcpy(tbc2.p, f.data(), f.size(), hipMemcpyHostToDevice));
+    }
     F.alloc(nd);
     TT_HIP(hipMemcpy(F.p, Fhost.data(), nd * 8, hipMemcpyHostToDevice));
     std::vector<double> Ph(nel);
@@ -136,6 +149,8 @@ Context::Context(const tt_params& p) : prm(p) {
     op_fine64.hs[0] = afh_scale[0];
     op_fine64.hs[1] = afh_scale[1];
     op_fine64.mask = mask.p;
+    op_fine64.tbc4 = tbc4.p;
+    op_fine64.tbc2 = tbc2.p;
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
 

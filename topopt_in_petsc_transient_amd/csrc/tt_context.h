@@ -17,7 +17,7 @@ struct Context;
 struct MGLevel {
     Grid g;
     bool fp64 = true;                  // precision of this level's vectors and element matrices
-    DBuf<uint8_t> mask;
+    DBuf<uint8_t> mask, tbc4, tbc2;    // node mask; per-tile flags of clamped nodes (8x8x4, 8x8x2 tiles)
     DBuf<double> kf64, mf64;           // FP64 coefficients (coarse levels: averages of children)
     DBuf<float> kf32, mf32;            // mixed mode, finest level
     DBuf<bf16_t> kfbf, mfbf;           // mixed mode, coarse levels
@@ -93,7 +93,7 @@ struct Context {
     DBuf<double> xphys, kf, mf, dkf, dmf, P, Pkf, Pmf;
     DBuf<double> rcnt;  // per node: sum of region indicators of the adjacent elements
     DBuf<double> F;     // load location vector (3 nn)
-    DBuf<uint8_t> mask;
+    DBuf<uint8_t> mask, tbc4, tbc2;
     std::vector<double> Fhost;
     OpLevel op_fine64{};  // FP64 fine operator view (for FCG residuals)
     DBuf<double> KE64d, M864d, AF64d;

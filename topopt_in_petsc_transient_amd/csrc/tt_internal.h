@@ -150,7 +150,12 @@ struct OpLevel {
     const float* AF32;
     const uint32_t* AFH;   // FP16-split fragments (3 x 2 x 64 x 4 words), K / M rows scaled by 1 / hs
     float hs[2];
+    const uint8_t* tbc4;   // per 8x8x4 / 8x8x2 tile: 1 if its staged node box holds a clamped node
+    const uint8_t* tbc2;
 };
+// Per-tile flags of 8 x 8 x tz node tiles: 1 if the tile's node box (one halo layer) holds a node
+// with a nonzero mask.
+void tile_bc_flags(const Grid& g, const std::vector<uint8_t>& mask, int tz, std::vector<uint8_t>& f);
 
 // MFMA A fragments of the interleaved [KE; M8 (x) I3] matrix, lane order (see tt_kernels.hip)
 // Symmetric coarsest solve on the packed lower triangle (64 x 64 tiles): size of the packed array,

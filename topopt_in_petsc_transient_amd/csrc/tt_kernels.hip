@@ -17,6 +17,15 @@
 namespace tt {
 
 // ------------------------------------------------------------------ helpers
+// Element i of p through a 32-bit byte offset from a uniform base (scalar-base addressing; every
+// array here is below 4 GiB, which the context checks at creation).
+template <typename T>
+__device__ __forceinline__ T ld(const T* p, unsigned i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T));
+}
+__device__ __forceinline__ double coef_ld(const double* p, unsigned i) { return ld(p, i); }
+__device__ __forceinline__ double coef_ld(const float* p, unsigned i) { return (double)ld(p, i); }
+__device__ __forceinline__ double coef_ld(const bf16_t* p, unsigned i) { return (double)__uint_as_float(((uint32_t)ld(p, i)) << 16); }
 __device__ __forceinline__ double coef_ld(const double* p, long i) { return p[i]; }
 __device__ __forceinline__ double coef_ld(const float* p, long i) { return (double)p[i]; }
 __device__ __forceinline__ double coef_ld(const bf16_t* p, long i) {
@@ -201,8 +210,8 @@ template <typename T>
 struct StageOne {  // w = u
     const T* u;
     __device__ __forceinline__ void prologue() {}
-    __device__ __forceinline__ T k(int i) const { return u[i]; }
-    __device__ __forceinline__ T m(int i) const { return u[i]; }
+    __device__ __forceinline__ T k(unsigned i) const { return ld(u, i); }
+    __device__ __forceinline__ T m(unsigned i) const { return ld(u, i); }
 };
 // Flexible-CG direction p = z + beta p_old, beta = z.(r - r_prev) / gamma_old computed by k_fcg_beta.
 template <typename T>
@@ -213,8 +222,8 @@ struct StageFcg {
     int first;
     T beta;
     __device__ __forceinline__ void prologue() { beta = first ? (T)0 : (T)scal[SC_BETA]; }
-    __device__ __forceinline__ T k(int i) const { return first ? z[i] : z[i] + beta * pold[i]; }
-    __device__ __forceinline__ T m(int i) const { return k(i); }
+    __device__ __forceinline__ T k(unsigned i) const { return first ? ld(z, i) : ld(z, i) + beta * ld(pold, i); }
+    __device__ __forceinline__ T m(unsigned i) const { return k(i); }
 };
 // First Chebyshev iterate x0 = c1 D^-1 b formed on the fly, with b = (T) r (FP64) on the finest level
 template <typename T, bool FROM64>
@@ -224,16 +233,16 @@ struct StageChebFirst {
     const T* dinv;
     T c1;
     __device__ __forceinline__ void prologue() {}
-    __device__ __forceinline__ T bval(int i) const { return FROM64 ? (T)r64[i] : b[i]; }
-    __device__ __forceinline__ T k(int i) const { return c1 * dinv[i] * bval(i); }
-    __device__ __forceinline__ T m(int i) const { return k(i); }
+    __device__ __forceinline__ T bval(unsigned i) const { return FROM64 ? (T)ld(r64, i) : ld(b, i); }
+    __device__ __forceinline__ T k(unsigned i) const { return c1 * ld(dinv, i) * bval(i); }
+    __device__ __forceinline__ T m(unsigned i) const { return k(i); }
 };
 struct StageThree {  // wK = sum_i cK_i in_i, wM = sum_i cM_i in_i (FP64)
     const double *u, *v, *a;
     double k0, k1, k2, m0, m1, m2;
     __device__ __forceinline__ void prologue() {}
-    __device__ __forceinline__ double k(int i) const { return k0 * u[i] + k1 * v[i] + k2 * a[i]; }
-    __device__ __forceinline__ double m(int i) const { return m0 * u[i] + m1 * v[i] + m2 * a[i]; }
+    __device__ __forceinline__ double k(unsigned i) const { return k0 * ld(u, i) + k1 * ld(v, i) + k2 * ld(a, i); }
+    __device__ __forceinline__ double m(unsigned i) const { return m0 * ld(u, i) + m1 * ld(v, i) + m2 * ld(a, i); }
 };
 
 template <typename T, typename CT, bool TWO, bool MASKIN, class Stage, class Epi>
@@ -579,6 +588,31 @@ static int device_cus() {
     return cus;
 }
 
+void tile_bc_flags(const Grid& g, const std::vector<uint8_t>& mask, int tz, std::vector<uint8_t>& f) {
+    const int tx = (g.nx + 7) / 8, ty = (g.ny + 7) / 8, tzn = (g.nz + tz - 1) / tz;
+    f.assign((size_t)tx * ty * tzn, 0);
+    // tile a (extent e) stages nodes e a - 1 .. e a + e
+    auto range = [](int i, int e, int nt, int& lo, int& hi) {
+        lo = std::max(0, (i - e) / e - 1);
+        hi = std::min(nt - 1, (i + 1) / e);
+    };
+    for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) {
+                if (!mask[i + (size_t)g.nx * (j + (size_t)g.ny * k)]) continue;
+                int a0, a1, b0, b1, c0, c1;
+                range(i, 8, tx, a0, a1);
+                range(j, 8, ty, b0, b1);
+                range(k, tz, tzn, c0, c1);
+                for (int c = c0; c <= c1; c++)
+                    for (int b = b0; b <= b1; b++)
+                        for (int a = a0; a <= a1; a++)
+                            if (i >= 8 * a - 1 && i <= 8 * a + 8 && j >= 8 * b - 1 && j <= 8 * b + 8 &&
+                                k >= tz * c - 1 && k <= tz * c + tz)
+                                f[a + (size_t)tx * (b + (size_t)ty * c)] = 1;
+            }
+}
+
 // Launch of k_apply_tiles: one block per tile, rounded up to a multiple of 8 for the XCD-aware
 // tile order; returns the block count (= number of reduction partials).
 template <typename T, typename CT, int TZ_, bool MASKIN, class Stage, class Epi>
@@ -587,7 +621,8 @@ static int launch_tiles_ct(const OpLevel& L, const T* AF, int tx, int ty, int tz
     const int ntiles = tx * ty * tz;
     const int nb = 8 * ((ntiles + 7) / 8);
     hipLaunchKernelGGL((k_apply_tiles<T, CT, TZ_, MASKIN, Stage, Epi>), dim3(nb), dim3(PTile<TZ_>::NTH), 0, s,
-                       gargs(L.g), tx, ty, ntiles, AF, (const CT*)L.kf, (const CT*)L.mf, cK, cM, L.mask, st, epi);
+                       gargs(L.g), tx, ty, ntiles, AF, (const CT*)L.kf, (const CT*)L.mf, cK, cM, L.mask,
+                       TZ_ == 4 ? L.tbc4 : L.tbc2, st, epi);
     TT_HIP(hipGetLastError());
     return nb;
 }

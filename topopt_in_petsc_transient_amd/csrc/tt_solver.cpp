@@ -60,6 +60,15 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
         build_mask(g, c->prm.boundaries, m);
         lev->mask.alloc(g.nn);
         TT_HIP(hipMemcpy(lev->mask.p, m.data(), g.nn, hipMemcpyHostToDevice));
+        {
+            std::vector<uint8_t> f;
+            tile_bc_flags(g, m, 4, f);
+            lev->tbc4.alloc(f.size());
+            TT_HIP(hipMemcpy(lev->tbc4.p, f.data(), f.size(), hipMemcpyHostToDevice));
+            tile_bc_flags(g, m, 2, f);
+            lev->tbc2.alloc(f.size());
+            TT_HIP(hipMemcpy(lev->tbc2.p, f.data(), f.size(), hipMemcpyHostToDevice));
+        }
         double ke[576], m8[64];
         hex8_stiffness(g.hx, g.hy, g.hz, c->prm.nu, ke);
         hex8_mass(g.hx, g.hy, g.hz, m8);
@@ -99,6 +108,8 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
             b->alloc(g.nd * es);
         lev->op.g = g;
         lev->op.mask = lev->mask.p;
+        lev->op.tbc4 = lev->tbc4.p;
+        lev->op.tbc2 = lev->tbc2.p;
         lev->op.KE64 = lev->KE64.p;
         lev->op.M864 = lev->M864.p;
         lev->op.KE32 = lev->KE32.p;
