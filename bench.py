@@ -33,9 +33,13 @@ METRIC = "Newmark steps/s + adjoint-gradient time, 128x64x64 hex8, mixed vs FP64
 PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 PEAK_FP64 = 78.6e12        # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_FP32 = 157.3e12       # MI355X dense FP32 matrix, flop/s
+PEAK_FP16 = 2.5e15         # MI355X dense FP16 matrix, flop/s
+# matrix instructions of one FP32 A apply at 128x64x64: 17 x 9 x 17 tiles of 8x8x4 nodes, 28 batches of
+# 16 elements per tile, 9 FP16 16x16x32 instructions (16,384 flops each) per batch
+MFMA_EXEC_A32 = 17 * 9 * 17 * 28 * 9 * 16384
 # HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
 # scaled by the counters' reading for a copy kernel of known size); None until measured.
-TRAFFIC_A32 = 20227778  # FETCH 6680.6 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (copy-calibrated)
+TRAFFIC_A32 = 20754932  # FETCH 6938.1 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (copy-calibrated)
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
@@ -225,7 +229,14 @@ def main():
             "frac": ach / (PEAK_FP32 / 1e12), "traffic": TRAFFIC_A32,
             "kernel": "k_apply_tiles<float,float,4,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
                       "128x64x64, 8x8x4-element tiles on MFMA 16x16x4)",
-            "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl}
+            "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl,
+            # the FP32-accurate contraction runs as three FP16 products on the matrix cores: executed
+            # matrix-instruction work per launch (2,601 tiles x 28 batches x 9 v_mfma_f32_16x16x32_f16
+            # of 16,384 flops) against the FP16 dense peak
+            "matrix_pipe": {"instr": "v_mfma_f32_16x16x32_f16", "executed_flops_per_launch": MFMA_EXEC_A32,
+                            "achieved": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12,
+                            "unit": "TFLOP/s",
+                            "frac": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / PEAK_FP16}}
     value = args.steps / mixed["seconds"]
     rec = {
         "metric": METRIC, "value": value, "unit": "Newmark steps/s", "n_gpus": 1, "steps": args.steps,
