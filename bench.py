@@ -141,6 +141,62 @@ def gradient_run(capi, x, precision, nsteps, slots):
             "solves": st.solves, "iters_per_solve": st.krylov_iters / max(1, st.solves)}, fx, df
 
 
+def step_roofline_ms(precision, iters, cheb=2):
+    """Roofline time of one Newmark step of the 128x64x64 workload: the sum over its kernels of
+    max(algorithmic flops / matrix peak, algorithmic bytes / HBM bandwidth), with the measured Krylov
+    iterations per solve.  Kernels: the fused right-hand side (K and M contractions of two state
+    combinations), the initial residual, the initial guess, the corrector; per iteration the FP64 CG
+    apply (stages z and p_old, writes p and q), the CG vector update, and the V-cycle (2 cheb operator
+    applies per smoothing level with their Chebyshev vector traffic, restriction and prolongation, the
+    dense coarsest solve on the packed inverse)."""
+    ex, ey, ez = NEL
+
+    def lvl(l):
+        f = 1 << l
+        nel = (ex // f) * (ey // f) * (ez // f)
+        nn = (ex // f + 1) * (ey // f + 1) * (ez // f + 1)
+        return nel, nn, 3 * nn
+
+    def t(flops, nbytes, peak):
+        return max(flops / peak, nbytes / PEAK_HBM)
+
+    nel0, nn0, nd0 = lvl(0)
+    c = 2 * 1152  # flops per element of the stacked [K; M] contraction
+    step = t(c * nel0, 8 * (5 * nd0 + 2 * nel0) + nn0, PEAK_FP64)  # right-hand side
+    step += t(c * nel0, 8 * (3 * nd0 + 2 * nel0) + nn0, PEAK_FP64)  # initial residual
+    step += t(4 * nd0, 8 * 3 * nd0, PEAK_FP64)  # b.b, r.r
+    step += t(4 * nd0, 8 * 4 * nd0, PEAK_FP64) + t(6 * nd0, 8 * 6 * nd0, PEAK_FP64)  # guess, corrector
+    it = t(c * nel0, 8 * (4 * nd0 + 2 * nel0) + nn0, PEAK_FP64)  # CG apply
+    it += t(6 * nd0, 8 * 7 * nd0, PEAK_FP64)  # CG update
+    mixed = precision == 1
+    for l in range(NLVLS - 1):
+        nel, nn, nd = lvl(l)
+        es = 4 if mixed else 8
+        ces = (4 if l == 0 else 2) if mixed else 8  # coefficient storage (FP32 / BF16 / FP64)
+        peak = PEAK_FP32 if mixed else PEAK_FP64
+        it += 2 * cheb * t(c * nel, es * 6 * nd + ces * 2 * nel + nn, peak)
+        _, _, ndc = lvl(l + 1)
+        it += t(54 * nd, es * (nd + ndc), peak) + t(16 * nd, es * (2 * nd + ndc), peak)  # restrict, prolong
+    _, _, nc = lvl(NLVLS - 1)
+    it += t(2 * nc * nc, 8 * nc * (nc + 1) // 2, PEAK_FP64)  # coarsest solve on the packed inverse
+    return 1e3 * (step + iters * it)
+
+
+def gradient_roofline_s(precision, forward_steps, adjoint_steps, iters):
+    """Roofline time of the adjoint gradient: forward steps (first sweep and revolve recomputation)
+    plus adjoint steps, each a solve of the same operator plus the split K / M apply of the previous
+    multiplier, the adjoint vector combination and the sensitivity contraction (two stacked GEMMs,
+    4 x 1152 flops per element, reading 7 dof vectors)."""
+    ex, ey, ez = NEL
+    nel = ex * ey * ez
+    nd = 3 * (ex + 1) * (ey + 1) * (ez + 1)
+    extra = (max(2 * 1152 * nel / PEAK_FP64, 8 * (3 * nd + 2 * nel) / PEAK_HBM)  # split apply
+             + 8 * 10 * nd / PEAK_HBM  # adjoint combination
+             + max(4 * 1152 * nel / PEAK_FP64, 8 * (7 * nd + 2 * nel) / PEAK_HBM))  # sensitivities
+    step = step_roofline_ms(precision, iters) * 1e-3
+    return forward_steps * step + adjoint_steps * (step + extra)
+
+
 def cpu_baseline(steps):
     """Oracle (FP64 C restatement, Jacobi-PCG) on the first `steps` Newmark steps of the same forward sweep."""
     from oracle import oracle as O
@@ -228,7 +284,7 @@ def main():
     roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": ach / (PEAK_FP32 / 1e12), "traffic": TRAFFIC_A32,
             "kernel": "k_apply_tiles<float,float,4,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
-                      "128x64x64, 8x8x4-element tiles on MFMA 16x16x4)",
+                      "128x64x64, 8x8x4-node tiles, FP16-split v_mfma_f32_16x16x32_f16)",
             "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl,
             # the FP32-accurate contraction runs as three FP16 products on the matrix cores: executed
             # matrix-instruction work per launch (2,601 tiles x 28 batches x 9 v_mfma_f32_16x16x32_f16
@@ -237,6 +293,19 @@ def main():
                             "achieved": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12,
                             "unit": "TFLOP/s",
                             "frac": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / PEAK_FP16}}
+    # whole-step and whole-gradient rooflines (algorithmic work x measured iterations, against the
+    # measured times)
+    steps_roof = {}
+    for name, run, prec in (("mixed", mixed, 1), ("fp64", fp64, 0)):
+        tr = step_roofline_ms(prec, run["iters_per_solve"], CHEB[prec])
+        tm = 1e3 * run["seconds"] / args.steps
+        steps_roof[name] = {"t_roof_ms": tr, "t_measured_ms": tm, "frac": tr / tm}
+    if grad:
+        for name, prec in (("mixed", 1), ("fp64", 0)):
+            g = grad[name]
+            tr = gradient_roofline_s(prec, g["forward_steps"], g["adjoint_steps"], g["iters_per_solve"])
+            steps_roof["gradient_" + name] = {"t_roof_s": tr, "t_measured_s": g["seconds"], "frac": tr / g["seconds"]}
+    roof["newmark_step"] = steps_roof
     value = args.steps / mixed["seconds"]
     rec = {
         "metric": METRIC, "value": value, "unit": "Newmark steps/s", "n_gpus": 1, "steps": args.steps,
