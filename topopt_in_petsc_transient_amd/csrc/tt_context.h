@@ -127,7 +127,9 @@ struct Context {
     DBuf<double> st_cur, st_next;
 
     // adjoint work
-    DBuf<double> lam, lamn, lamn2, lamn3, lamT, mu, nu, mun, nun, hdu, hddu, hdddu, Kl, Ml, rhs, df, fxs, nu0;
+    DBuf<double> lam, lamT,
+        lamh[6],  // multipliers of the later steps t+1 .. t+6 (the adjoint solves' initial guess)
+        mu, nu, mun, nun, hdu, hddu, hdddu, Kl, Ml, rhs, df, fxs, nu0;
 
     // PDE filter
     double KF[64];

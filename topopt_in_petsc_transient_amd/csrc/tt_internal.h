@@ -306,5 +306,12 @@ void launch_lincomb(double a, const double* x, double b, const double* y, double
 // out = a x + b y + c z
 void launch_lincomb3(double a, const double* x, double b, const double* y, double c, const double* z, double* out,
                      long n, hipStream_t s);
+// out = sum_k c[k] v[k] over the m <= 8 vectors of e (polynomial extrapolation of a sequence)
+struct Extrap {
+    const double* v[8];
+    double c[8];
+    int m;
+};
+void launch_extrapolate(const Extrap& e, double* out, long n, hipStream_t s);
 
 }  // namespace tt

@@ -580,5 +580,16 @@ void launch_lincomb3(double a, const double* x, double b, const double* y, doubl
     hipLaunchKernelGGL(k_lincomb3, dim3(nblk(n)), dim3(BS2), 0, s, a, x, b, y, c, z, out, n);
     TT_HIP(hipGetLastError());
 }
+__global__ void k_extrapolate(Extrap e, double* out, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = e.c[0] * e.v[0][i];
+    for (int k = 1; k < e.m; k++) s = fma(e.c[k], e.v[k][i], s);
+    out[i] = s;
+}
+void launch_extrapolate(const Extrap& e, double* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(BS2), 0, s, e, out, n);
+    TT_HIP(hipGetLastError());
+}
 
 }  // namespace tt

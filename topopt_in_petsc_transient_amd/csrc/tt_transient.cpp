@@ -206,6 +206,12 @@ void Context::forward() {
 }
 
 // ------------------------------------------------------------------ adjoint
+// Multipliers the adjoint solves extrapolate from: the initial guess is the degree-5 polynomial through
+// the latest 6 of them (measured on the 128x64x64 gradient: 7.4 adjoint iterations per solve with 3
+// multipliers, 5.5 with 4, 4.6 with 5, 3.7 with 6, no gain beyond).
+constexpr int kAdjHist = 6;
+static_assert(kAdjHist <= (int)(sizeof(Context::lamh) / sizeof(Context::lamh[0])) && kAdjHist <= 8, "history size");
+
 void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     TT_CHECK(forward_done, TT_ERR_STATE, "tt_forward must be called first");
     const int T = prm.nsteps;
@@ -270,7 +276,7 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     };
 
     // one backward step at t: needs state t and state t-1
-    int nhist = 0;  // multipliers of later steps held in lamn, lamn2, lamn3
+    int nhist = 0;  // multipliers of later steps held in lamh[0..] (lamh[0] = lambda_{t+1})
     auto adjoint_step = [&](int t, const double* st, const double* stb) {
         grad_state(st);
         if (t == T) {
@@ -278,28 +284,35 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
                                    nu.p, rhs.p, nn, stream);
             // the terminal multiplier of the previous call is the initial guess (:877-883)
         } else {
-            launch_apply_split(op_fine64, 1.0, 1.0, lamn.p, Kl.p, Ml.p, stream);
+            launch_apply_split(op_fine64, 1.0, 1.0, lamh[0].p, Kl.p, Ml.p, stream);
             launch_adjoint_combine(Kl.p, Ml.p, hdu.p, hddu.p, hdddu.p, mun.p, nun.p, mask.p, coef, false, mu.p, nu.p,
                                    rhs.p, nn, stream);
-            // Initial guess: quadratic extrapolation of lambda_{t+1}, lambda_{t+2}, lambda_{t+3} (linear,
-            // then constant, near the end).  The reference starts from lambda_{t+1} (:917); the converged
-            // multiplier is the same to the solver tolerance.
-            if (nhist >= 3) launch_lincomb3(3.0, lamn.p, -3.0, lamn2.p, 1.0, lamn3.p, lam.p, nd, stream);
-            else if (nhist == 2) launch_lincomb(2.0, lamn.p, -1.0, lamn2.p, lam.p, nd, stream);
-            else launch_copy(lamn.p, lam.p, nd, true, true, stream);
+            // Initial guess: the polynomial through the last m = min(nhist, kAdjHist) multipliers,
+            // evaluated one step on: sum_k (-1)^(k+1) C(m, k) lambda_{t+k}.  The reference starts from
+            // lambda_{t+1} (:917); the converged multiplier is the same to the solver tolerance.
+            Extrap e{};
+            e.m = std::min(nhist, kAdjHist);
+            long bin = 1;
+            for (int k = 1; k <= e.m; k++) {
+                bin = bin * (e.m - k + 1) / k;
+                e.v[k - 1] = lamh[k - 1].p;
+                e.c[k - 1] = (k % 2 ? 1.0 : -1.0) * (double)bin;
+            }
+            launch_extrapolate(e, lam.p, nd, stream);
         }
         solve_A(rhs.p, lam.p);
         sens(0, lam.p, st, stb, t);
         // lambda_t becomes lambda_{t+1} of the next (earlier) step
-        std::swap(lamn3.p, lamn2.p);
-        std::swap(lamn2.p, lamn.p);
-        std::swap(lamn.p, lam.p);
-        nhist = std::min(nhist + 1, 3);
+        double* oldest = lamh[kAdjHist - 1].p;
+        for (int k = kAdjHist - 1; k > 0; k--) lamh[k].p = lamh[k - 1].p;
+        lamh[0].p = lam.p;
+        lam.p = oldest;
+        nhist = std::min(nhist + 1, kAdjHist);
         std::swap(mu.p, mun.p);
         std::swap(nu.p, nun.p);
         stats.adjoint_steps++;
         // keep the terminal multiplier as next call's starting guess
-        if (t == T) launch_copy(lamn.p, lamT.p, nd, true, true, stream);
+        if (t == T) launch_copy(lamh[0].p, lamT.p, nd, true, true, stream);
     };
 
     launch_copy(lamT.p, lam.p, nd, true, true, stream);
@@ -318,7 +331,7 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     // t = 0 (SolveLagrange0, :678-804; ComputeSensitivitiesElementLoop0, :2080-2204)
     const double* s0 = slots[0].p;
     grad_state(s0);
-    launch_apply_split(op_fine64, 1.0, 1.0, lamn.p, Kl.p, Ml.p, stream);
+    launch_apply_split(op_fine64, 1.0, 1.0, lamh[0].p, Kl.p, Ml.p, stream);
     launch_adjoint0_rhs(Kl.p, Ml.p, hdddu.p, mun.p, nun.p, mask.p, coef, rhs.p, nn, stream);
     solve_M(rhs.p, nu0.p);
     sens(1, nu0.p, s0, nullptr, 0);
