@@ -120,7 +120,11 @@ struct Context {
     std::map<std::vector<uintptr_t>, IterGraph> iter_graphs;
     void clear_graphs();  // (the multigrid set-up changes baked-in Chebyshev scalars)
 
-    // time states: slot storage (u, v, a contiguous, 3*nd each)
+    // time states: slot storage, kStateVecs x nd contiguous each: u, v, a and the kFwdHist - 1 previous
+    // accelerations (the Newmark solve's initial guess extrapolates the acceleration from them, so a
+    // state restored from a checkpoint continues exactly as in the first sweep)
+    static constexpr int kFwdHist = 4;
+    static constexpr int kStateVecs = 2 + kFwdHist;
     std::vector<DBuf<double>> slots;
     std::vector<int> slot_step;  // time step held by each slot (-1 = free)
     bool store_all = false;
