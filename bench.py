@@ -39,7 +39,7 @@ PEAK_FP16 = 2.5e15         # MI355X dense FP16 matrix, flop/s
 MFMA_EXEC_A32 = 17 * 9 * 17 * 28 * 9 * 16384
 # HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
 # scaled by the counters' reading for a copy kernel of known size); None until measured.
-TRAFFIC_A32 = 20754932  # FETCH 6938.1 KiB x 1.99922 + WRITE 6397.7 KiB x 1.0 (copy-calibrated)
+TRAFFIC_A32 = 20751964  # FETCH 6936.6 KiB x 1.99922 + WRITE 6397.8 KiB x 1.0 (copy-calibrated)
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
