@@ -135,14 +135,21 @@ def test_other_objectives_fp64(objective):
     assert rel(df, df_ref) < FP64_TOL
 
 
-def test_revolve_matches_store_all_bitwise():
+# Newmark steps of forward sweep + gradient for 30 steps: the 30 steps of the sweep plus the binomial
+# recomputation (the adjoint step at t reuses state t from the step at t + 1, so it adds none).
+# 29 slots hold every split point of the first descent: nothing is recomputed.
+REVOLVE_FORWARD_STEPS = {2: 127, 4: 70, 29: 30}
+
+
+@pytest.mark.parametrize("slots", sorted(REVOLVE_FORWARD_STEPS))
+def test_revolve_matches_store_all_bitwise(slots):
     _, pg = problem(nsteps=30)
     x = random_density(pg.nelx * pg.nely * pg.nelz, seed=6)
     c = _ctx(pg, x)
     c.complete_initial_state()
     c.forward()
     fx0, df0 = c.objective_gradient()
-    _, pg2 = problem(nsteps=30, slots=4)
+    _, pg2 = problem(nsteps=30, slots=slots)
     c2 = _ctx(pg2, x)
     c2.complete_initial_state()
     c2.forward()
@@ -150,7 +157,11 @@ def test_revolve_matches_store_all_bitwise():
     s = c2.stats()
     assert fx1 == fx0
     assert np.array_equal(df1, df0)
-    assert s.adjoint_steps == 30 and s.forward_steps > 30
+    assert s.adjoint_steps == 30 and s.forward_steps == REVOLVE_FORWARD_STEPS[slots]
+    # a second gradient after the same forward sweep starts again from the kept state T
+    fx2, df2 = c2.objective_gradient()
+    assert fx2 == fx0
+    assert np.array_equal(df2, df0)
 
 
 def test_mixed_precision_against_oracle():

@@ -167,7 +167,7 @@ Context::Context(const tt_params& p) : prm(p) {
     store_all = (nslots == T + 1);
     size_t freeb = 0, totalb = 0;
     TT_HIP(hipMemGetInfo(&freeb, &totalb));
-    const double need = (double)nslots * kStateVecs * nd * 8;
+    const double need = (double)(nslots + (store_all ? 2 : 4)) * kStateVecs * nd * 8;
     TT_CHECK(need < 0.85 * (double)freeb, TT_ERR_ARG,
              "state storage exceeds device memory: lower checkpoint_slots");
     slots.resize(nslots);
@@ -175,6 +175,10 @@ Context::Context(const tt_params& p) : prm(p) {
     slot_step.assign(nslots, -1);
     st_cur.alloc((size_t)kStateVecs * nd);
     st_next.alloc((size_t)kStateVecs * nd);
+    if (!store_all) {
+        st_last.alloc((size_t)kStateVecs * nd);
+        st_hi.alloc((size_t)kStateVecs * nd);
+    }
 
     // PDE filter (PDEFilter.cc:28-191) on the same node grid
     fg = g;

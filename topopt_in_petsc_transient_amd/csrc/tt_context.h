@@ -42,6 +42,7 @@ struct Multigrid {
     int nc = 0;
     bool dense = true;
     DBuf<double> Ainv, AinvP, sprow, spcol, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest level
+    DBuf<float> AinvP32;  // the packed inverse rounded to FP32, for an FP32 hierarchy
     DBuf<int> dinfo;
     void* blas = nullptr;  // rocblas_handle used by the rocSOLVER factorisation of the coarsest operator
     int coarse_deg = 12;
@@ -129,6 +130,9 @@ struct Context {
     std::vector<int> slot_step;  // time step held by each slot (-1 = free)
     bool store_all = false;
     DBuf<double> st_cur, st_next;
+    // checkpointed runs: st_last = state T as the forward sweep left it; st_hi = the later of the two
+    // states an adjoint step reads, kept from the previous (later) adjoint step
+    DBuf<double> st_last, st_hi;
 
     // adjoint work
     DBuf<double> lam, lamT,
@@ -148,12 +152,17 @@ struct Context {
 
     // helpers
     OpLevel fine_op(bool fp64) const;
-    SolveResult solve_A(const double* b, double* x);         // Newmark operator, MG-preconditioned FCG
-    SolveResult solve_M(const double* b, double* x);         // mass operator N M N + I - N, Jacobi FCG
-    SolveResult fcg(double cK, double cM, bool use_mg, const double* b, double* x);
-    int fcg_hint[2] = {0, 0};  // iterations of the last multigrid / Jacobi preconditioned solve
-    // sums of two single-value partial regions with one host round trip
-    void reduce2(int r1, int r2, double& a, double& b);
+    // Solves of one kind (forward Newmark, adjoint, mass) have similar iteration counts; the last few
+    // counts of each kind decide when the host starts reading the residual back.
+    enum SolveKind { kSolveForward = 0, kSolveAdjoint = 1, kSolveMass = 2, kSolveKinds = 3 };
+    static constexpr int kHintLen = 4;
+    SolveResult solve_A(const double* b, double* x, SolveKind kind);  // Newmark operator, MG-preconditioned FCG
+    SolveResult solve_M(const double* b, double* x);                  // mass operator N M N + I - N, Jacobi FCG
+    SolveResult fcg(double cK, double cM, bool use_mg, SolveKind kind, const double* b, double* x);
+    int fcg_hist[kSolveKinds][kHintLen] = {};  // iteration counts of the last solves of each kind (0: none)
+    // sums of two single-value partial regions with one host round trip; after1 (optional) receives
+    // the value stored right after region r1's partials
+    void reduce2(int r1, int r2, double& a, double& b, double* after1 = nullptr);
     void sync();
     double read_scal(int slot);
     double reduce1(int region);  // host sum of a partial region (synchronises)

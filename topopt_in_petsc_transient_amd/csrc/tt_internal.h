@@ -128,6 +128,7 @@ enum {
     SC_DONE,        // flexible CG: 1 once the recursive residual met the tolerance (x is then frozen)
     SC_TOL2,        // flexible CG: (rtol ||b||)^2
     SC_ZERO,        // flexible CG: 1 if b = 0 (the solution is x = 0)
+    SC_ITERS,       // flexible CG: iterations that moved x in this solve
     SC_FX0,         // objective accumulators start here
     SC_COUNT = 32
 };
@@ -163,6 +164,7 @@ void tile_bc_flags(const Grid& g, const std::vector<uint8_t>& mask, int tz, std:
 long sym_packed_size(int n);
 void launch_sym_pack(const double* A, int n, double* P, hipStream_t s);
 void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
+void launch_sym_solve(const float* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
 // Column-major lower triangle of an n x n matrix copied into its upper triangle.
 void launch_mirror_lower(double* A, int n, hipStream_t s);
 constexpr int AF32_WORDS = 18 * 64 + 3 * 2 * 64 * 4;  // FP32 fragments followed by the FP16-split ones
@@ -228,7 +230,9 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
                      hipStream_t s);
 // Flexible-CG (re)start from the b.b and r.r partial regions: device tolerance (rtol ||b||)^2, the
 // b = 0 flag, and whether the current x already meets the tolerance.
-void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, hipStream_t s);
+// reset_iters: a new solve (SC_ITERS = 0) rather than a restart of the same one.  After every
+// launch_fcg_update the PB_RR region holds nblk partials of r.r followed by SC_ITERS.
+void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, bool reset_iters, hipStream_t s);
 // x += alpha p, r -= alpha q with alpha = gamma / p.q, unless x already met the tolerance (set at the
 // (re)start, or by the previous update's r.r after its first iteration): then alpha = 0 from there on,
 // so the result does not depend on when the host looks at the residual.  x = 0 if b = 0.
@@ -239,9 +243,11 @@ void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipSt
 void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s);
 void launch_scale_by_norm(void* x, long n, Reducer& red, bool fp64, hipStream_t s);
 
-// Newmark corrector: vn = a0 v + a1 a + a2 (un - u); an = -a3 v - a4 a + a5 (un - u)
+// Newmark corrector: vn = a0 v + a1 a + a2 (un - u); an = -a3 v - a4 a + a5 (un - u).  With nh > 0 it
+// also writes the nh vectors hout[0] = a, hout[j] = hin[j - 1] (each n long, stored n apart).
 void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,
-                           double* an, const double* c, long n, hipStream_t s);
+                           double* an, const double* c, long n, const double* hin, double* hout, int nh,
+                           hipStream_t s);
 // adjoint combination (see transient.cpp)
 void launch_adjoint_combine(const double* Kl, const double* Ml, const double* hdu, const double* hddu,
                             const double* hdddu, const double* mun, const double* nun, const uint8_t* mask,

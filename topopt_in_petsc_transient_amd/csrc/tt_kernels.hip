@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "tt_internal.h"
 
@@ -1139,10 +1140,11 @@ void launch_sym_pack(const double* A, int n, double* P, hipStream_t s) {
     hipLaunchKernelGGL(k_sym_pack, dim3(nt, nt), dim3(256), 0, s, A, n, P);
     TT_HIP(hipGetLastError());
 }
-// per tile: row sums A_IJ x_J and (off the diagonal) column sums A_IJ^T x_I, written as partials.
-// Each thread first loads its 16 tile values (two per 16-byte load, all in flight), then the tile
-// goes through LDS once.
-__global__ __launch_bounds__(256) void k_sym_tiles(const double* __restrict__ P, const double* __restrict__ x, int n,
+// per tile: row sums A_IJ x_J and (off the diagonal) column sums A_IJ^T x_I, written as FP64
+// partials.  Each thread first loads its 16 tile values (16-byte loads, all in flight), then the
+// tile goes through LDS once.  PT = double, or float for the FP32 preconditioner (half the bytes).
+template <typename PT>
+__global__ __launch_bounds__(256) void k_sym_tiles(const PT* __restrict__ P, const double* __restrict__ x, int n,
                                                    double* __restrict__ prow, double* __restrict__ pcol) {
     __shared__ double tile[ST][ST + 1];
     __shared__ double xs[2][ST];
@@ -1152,21 +1154,24 @@ __global__ __launch_bounds__(256) void k_sym_tiles(const double* __restrict__ P,
     while ((I + 1) * (I + 2) / 2 <= id) I++;
     while (I * (I + 1) / 2 > id) I--;
     const int J = id - I * (I + 1) / 2;
-    const double2* T2 = reinterpret_cast<const double2*>(P + (size_t)id * ST * ST);
+    constexpr int VW = 16 / sizeof(PT), NL = ST * ST / (256 * VW);  // values per load, loads per thread
+    using V = typename std::conditional<VW == 2, double2, float4>::type;
+    const V* T2 = reinterpret_cast<const V*>(P + (size_t)id * ST * ST);
     const int tid = threadIdx.x;
-    double2 v[8];
+    V v[NL];
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = T2[tid + 256 * k];
+    for (int k = 0; k < NL; k++) v[k] = T2[tid + 256 * k];
     if (tid < ST) {
         const int j = J * ST + tid, i = I * ST + tid;
         xs[0][tid] = j < n ? x[j] : 0.0;
         xs[1][tid] = i < n ? x[i] : 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int e = 2 * (tid + 256 * k);
-        tile[e / ST][e % ST] = v[k].x;
-        tile[e / ST][e % ST + 1] = v[k].y;
+    for (int k = 0; k < NL; k++) {
+        const int e = VW * (tid + 256 * k);
+        const PT* pv = reinterpret_cast<const PT*>(&v[k]);
+#pragma unroll
+        for (int q = 0; q < VW; q++) tile[e / ST][e % ST + q] = (double)pv[q];
     }
     __syncthreads();
     const int part = tid >> 6, k = tid & 63;  // 4 x 64 threads: rows (parts 0, 1) and columns (parts 2, 3)
@@ -1198,13 +1203,21 @@ __global__ __launch_bounds__(256) void k_sym_sum(const double* __restrict__ prow
     const int i = I * ST + threadIdx.x;
     if (threadIdx.x < ST && i < n) y[i] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
-void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol,
-                      hipStream_t s) {
+template <typename PT>
+static void sym_solve(const PT* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s) {
     const int nt = sym_tiles(n);
-    hipLaunchKernelGGL(k_sym_tiles, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, P, b, n, prow, pcol);
+    hipLaunchKernelGGL(k_sym_tiles<PT>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, P, b, n, prow, pcol);
     TT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sym_sum, dim3(nt), dim3(256), 0, s, prow, pcol, n, nt, x);
     TT_HIP(hipGetLastError());
+}
+void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol,
+                      hipStream_t s) {
+    sym_solve(P, b, x, n, prow, pcol, s);
+}
+void launch_sym_solve(const float* P, const double* b, double* x, int n, double* prow, double* pcol,
+                      hipStream_t s) {
+    sym_solve(P, b, x, n, prow, pcol, s);
 }
 
 // ------------------------------------------------------------------ reductions / Krylov
@@ -1262,7 +1275,7 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
 // Start (or restart) of a flexible-CG solve, from the partials of b.b and of r.r for the current
 // x: the tolerance (rtol ||b||)^2, whether b = 0, and whether x already meets the tolerance.
 __global__ void k_fcg_init(const double* __restrict__ bb_part, int bb_nblk, const double* __restrict__ rr_part,
-                           int rr_nblk, double rtol2, double* scal) {
+                           int rr_nblk, double rtol2, int reset_iters, double* scal) {
     double bb[1], rr[1];
     sum_partials<1>(bb_part, bb_nblk, bb);
     sum_partials<1>(rr_part, rr_nblk, rr);
@@ -1271,11 +1284,12 @@ __global__ void k_fcg_init(const double* __restrict__ bb_part, int bb_nblk, cons
         scal[SC_TOL2] = tol2;
         scal[SC_ZERO] = bb[0] == 0.0 ? 1.0 : 0.0;
         scal[SC_DONE] = (bb[0] == 0.0 || rr[0] <= tol2) ? 1.0 : 0.0;
+        if (reset_iters) scal[SC_ITERS] = 0.0;
     }
 }
-void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, hipStream_t s) {
+void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, bool reset_iters, hipStream_t s) {
     hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(BS), 0, s, red.part(bb_region), red.nblk[bb_region],
-                       red.part(rr_region), red.nblk[rr_region], rtol2, red.scal.p);
+                       red.part(rr_region), red.nblk[rr_region], rtol2, reset_iters ? 1 : 0, red.scal.p);
     TT_HIP(hipGetLastError());
 }
 
@@ -1294,6 +1308,7 @@ __global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, con
         scal[SC_DONE] = done ? 1.0 : 0.0;
         scal[SC_ALPHA] = done ? 0.0 : gam / pq[0];
         scal[SC_GAMMA_OLD] = gam;
+        if (!done) scal[SC_ITERS] += 1.0;
     }
 }
 
@@ -1315,6 +1330,7 @@ __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ 
         acc[0] = fma(rn, rn, acc[0]);
     }
     write_partials<1>(acc, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) part[gridDim.x] = scal[SC_ITERS];  // read with the partials
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        bool first, hipStream_t s) {
@@ -1393,13 +1409,16 @@ void launch_scale(double* x, long n, double a, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ Newmark / adjoint vector kernels
-// corrector (LinearElasticity.cc:1363-1375): c = a[0..5]
+// corrector (LinearElasticity.cc:1363-1375): c = a[0..5].  The acceleration history of the new state
+// is shifted in the same pass: hout[0] = a, hout[j] = hin[j - 1] (j < nh), vectors n apart.
 __global__ void k_newmark_update(const double* __restrict__ u, const double* __restrict__ v,
                                  const double* __restrict__ a, const double* __restrict__ un, double* __restrict__ vn,
                                  double* __restrict__ an, double a0, double a1, double a2, double a3, double a4,
-                                 double a5, long n) {
+                                 double a5, long n, const double* __restrict__ hin, double* __restrict__ hout, int nh) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (nh > 0) hout[i] = a[i];
+    for (int j = 1; j < nh; j++) hout[j * n + i] = hin[(j - 1) * n + i];
     const double du = un[i] - u[i];
     double vv = 0.0;
     vv += a0 * v[i];
@@ -1413,9 +1432,10 @@ __global__ void k_newmark_update(const double* __restrict__ u, const double* __r
     an[i] = aa;
 }
 void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,
-                           double* an, const double* c, long n, hipStream_t s) {
+                           double* an, const double* c, long n, const double* hin, double* hout, int nh,
+                           hipStream_t s) {
     hipLaunchKernelGGL(k_newmark_update, dim3(nblocks(n)), dim3(BS), 0, s, u, v, a, un, vn, an, c[0], c[1], c[2],
-                       c[3], c[4], c[5], n);
+                       c[3], c[4], c[5], n, hin, hout, nh);
     TT_HIP(hipGetLastError());
 }
 

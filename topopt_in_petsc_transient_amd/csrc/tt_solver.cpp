@@ -127,6 +127,7 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
     if (dense) {
         Ainv.alloc((size_t)nc * nc);
         AinvP.alloc(sym_packed_size(nc));
+        if (!L.back()->fp64) AinvP32.alloc(sym_packed_size(nc));
         {
             const long nt = (nc + 63) / 64, ntiles = nt * (nt + 1) / 2;
             sprow.alloc(ntiles * 64);
@@ -271,6 +272,7 @@ void Multigrid::assemble_coarse_inverse() {
     TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
     launch_mirror_lower(Ainv.p, n, s);
     launch_sym_pack(Ainv.p, n, AinvP.p, s);
+    if (AinvP32.p) launch_copy(AinvP.p, AinvP32.p, sym_packed_size(n), true, false, s);
     TT_HIP(hipStreamSynchronize(s));
 }
 
@@ -333,6 +335,8 @@ OpLevel Multigrid::coarse_op64() const {
 }
 
 // direct64: the FP32 level above restricted straight into cb and prolongs straight from cx.
+// An FP32 hierarchy applies the inverse rounded to FP32 (half the bytes; the products and their
+// sums stay FP64): measured on the 128x64x64 Newmark sweeps, the Krylov iteration counts do not change.
 void Multigrid::coarse_solve(bool direct64) {
     MGLevel& lc = *L.back();
     hipStream_t s = ctx->stream;
@@ -341,8 +345,10 @@ void Multigrid::coarse_solve(bool direct64) {
     if (!lc.fp64) {
         b64 = cb.p;
         x64 = cx.p;
-        if (direct64) {
-            launch_sym_solve(AinvP.p, b64, x64, nc, sprow.p, spcol.p, s);
+        if (dense) {
+            if (!direct64) launch_copy(lc.b.p, cb.p, nc, false, true, s);
+            launch_sym_solve(AinvP32.p, b64, x64, nc, sprow.p, spcol.p, s);
+            if (!direct64) launch_copy(x64, lc.x.p, nc, true, false, s);
             return;
         }
         launch_copy(lc.b.p, cb.p, nc, false, true, s);
@@ -465,15 +471,17 @@ void Multigrid::apply(const double* r, double* z) {
 // ------------------------------------------------------------------ context-level Krylov
 void Context::sync() { TT_HIP(hipStreamSynchronize(stream)); }
 
-void Context::reduce2(int r1, int r2, double& a, double& b) {
+void Context::reduce2(int r1, int r2, double& a, double& b, double* after1) {
     const int n1 = red.nblk[r1] * red.nv[r1], n2 = red.nblk[r2] * red.nv[r2];
-    TT_HIP(hipMemcpyAsync(red.hpart, red.part(r1), sizeof(double) * n1, hipMemcpyDeviceToHost, stream));
-    TT_HIP(hipMemcpyAsync(red.hpart + n1, red.part(r2), sizeof(double) * n2, hipMemcpyDeviceToHost, stream));
+    const int c1 = n1 + (after1 ? 1 : 0);
+    TT_HIP(hipMemcpyAsync(red.hpart, red.part(r1), sizeof(double) * c1, hipMemcpyDeviceToHost, stream));
+    TT_HIP(hipMemcpyAsync(red.hpart + c1, red.part(r2), sizeof(double) * n2, hipMemcpyDeviceToHost, stream));
     TT_HIP(hipStreamSynchronize(stream));
     a = 0.0;
     b = 0.0;
     for (int i = 0; i < n1; i++) a += red.hpart[i];
-    for (int i = 0; i < n2; i++) b += red.hpart[n1 + i];
+    for (int i = 0; i < n2; i++) b += red.hpart[c1 + i];
+    if (after1) *after1 = red.hpart[n1];
 }
 
 double Context::read_scal(int slot) {
@@ -498,7 +506,7 @@ OpLevel Context::fine_op(bool fp64) const {
     return o;
 }
 
-SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, double* x) {
+SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, const double* b, double* x) {
     const long nd = g.nd;
     const OpLevel& op = op_fine64;
     const double rtol = prm.rtol;
@@ -507,18 +515,25 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
     launch_residual(op, cK, cM, b, x, kr.p, true, stream);
     launch_dot(b, b, nd, red, PB_MISC, stream);
     launch_dot(kr.p, kr.p, nd, red, PB_MISC2, stream);
-    launch_fcg_init(red, PB_MISC, PB_MISC2, rtol * rtol, stream);
-    // The previous solve with this preconditioner took `hint` iterations: the residual is read back
-    // (a host round trip) only from iteration hint - 2 on.  The device freezes x at the first
-    // iteration whose recursive residual meets the tolerance (k_fcg_alpha), so a late look only
-    // costs idle iterations, never a different answer.
-    int& hint = fcg_hint[use_mg ? 0 : 1];
+    launch_fcg_init(red, PB_MISC, PB_MISC2, rtol * rtol, true, stream);
+    // The residual is read back (a host round trip, ~25 us of idle GPU when the solve goes on) only
+    // from one iteration before the fewest any of the last kHintLen solves of this kind took.  A look
+    // that comes too late costs idle iterations instead (~0.4 ms each at 128x64x64), never a
+    // different answer: the device freezes x at the first iteration whose recursive residual meets
+    // the tolerance (k_fcg_alpha).  Measured on the 128x64x64 forward sweep (2-3 iterations per
+    // solve, a few solves converging after one): looking first at the fewest instead of one before
+    // it ran 3 % slower.
+    int* hist = fcg_hist[kind];
+    int first_look = 1 << 30;
+    for (int k = 0; k < kHintLen; k++) first_look = std::min(first_look, hist[k]);
+    first_look = std::max(first_look - 1, 1);
     // The recursive residual of this FP64 recursion drifts from b - A x by rounding only (~1e-14 of
     // ||b|| here); the true residual is confirmed when the tolerance comes within reach of that.
     const bool confirm = rtol < 1e-9;
     SolveResult res;
     stats.solves++;
     double rel = 0.0, bb = 0.0;
+    int moved = 0;  // iterations that moved x, as the device counted them at the last look
     bool first = true, conv = false;
     double* pold = kp0.p;
     double* pnew = kp1.p;
@@ -581,9 +596,10 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
         std::swap(pold, pnew);
         first = false;
         it++;
-        if (it < hint - 2) continue;
-        double rr = 0.0;
-        reduce2(PB_RR, PB_MISC, rr, bb);  // PB_MISC holds the b.b partials for the whole solve
+        if (it < first_look) continue;
+        double rr = 0.0, worked = 0.0;
+        reduce2(PB_RR, PB_MISC, rr, bb, &worked);  // PB_MISC holds the b.b partials for the whole solve
+        moved = (int)worked;
         if (!std::isfinite(rr) || !std::isfinite(bb))
             throw Error{TT_ERR_NONFINITE, "non-finite right-hand side or residual in flexible CG"};
         if (bb == 0.0) {  // b = 0: the device has set x = 0
@@ -608,12 +624,15 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
                 conv = true;
             } else {
                 first = true;
-                launch_fcg_init(red, PB_MISC, PB_MISC2, rtol * rtol, stream);
+                launch_fcg_init(red, PB_MISC, PB_MISC2, rtol * rtol, false, stream);
             }
         }
     }
     if (graphed) launch_copy(xs, x, nd, true, true, stream);
-    hint = it;
+    // the count of the device, not `it`: a solve frozen before the first look would otherwise keep
+    // the first look where it is even when later solves need fewer iterations
+    for (int k = kHintLen - 1; k > 0; k--) hist[k] = hist[k - 1];
+    hist[0] = conv ? std::max(moved, 1) : it;
     stats.krylov_iters += it;
     stats.last_iters = it;
     stats.last_relres = rel;
@@ -627,7 +646,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, const double* b, dou
     return res;
 }
 
-SolveResult Context::solve_A(const double* b, double* x) { return fcg(cK_A, cM_A, true, b, x); }
-SolveResult Context::solve_M(const double* b, double* x) { return fcg(0.0, 1.0, false, b, x); }
+SolveResult Context::solve_A(const double* b, double* x, SolveKind kind) { return fcg(cK_A, cM_A, true, kind, b, x); }
+SolveResult Context::solve_M(const double* b, double* x) { return fcg(0.0, 1.0, false, kSolveMass, b, x); }
 
 }  // namespace tt

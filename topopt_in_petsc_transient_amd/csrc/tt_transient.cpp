@@ -89,10 +89,9 @@ void Context::newmark_step(const double* s_in, double* s_out, int t) {
         e.c[2] += a[4] / a[5];
         launch_extrapolate(e, un, nd, stream);
     }
-    solve_A(rhs.p, un);
-    launch_newmark_update(u, v, ac, un, vn, an, a, nd, stream);
-    // acceleration history of the new state: a_{t-1} .. a_{t-kFwdHist+1}
-    if (kFwdHist > 1) launch_copy(s_in + 2 * nd, s_out + 3 * nd, (long)(kFwdHist - 1) * nd, true, true, stream);
+    solve_A(rhs.p, un, kSolveForward);
+    // corrector, and the acceleration history of the new state: a_{t-1} .. a_{t-kFwdHist+1}
+    launch_newmark_update(u, v, ac, un, vn, an, a, nd, s_in + 3 * nd, s_out + 3 * nd, kFwdHist - 1, stream);
     stats.forward_steps++;
 }
 
@@ -118,7 +117,7 @@ static int revolve_right(int n, int s) {
 
 struct Revolve {
     Context* c;
-    std::function<void(int)> adjoint_at;  // working state st_cur holds state t-1
+    std::function<void(int)> adjoint_at;  // adjoint step t; the working state st_cur holds state t-1
     int find_slot(int step) const {
         for (size_t i = 0; i < c->slot_step.size(); i++)
             if (c->slot_step[i] == step) return (int)i;
@@ -220,6 +219,7 @@ void Context::forward() {
                 k++;
             }
         }
+        launch_copy(st_cur.p, st_last.p, nd3, true, true, stream);  // state T, read by the first adjoint step
     }
     sync();
     forward_done = true;
@@ -321,7 +321,7 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
             }
             launch_extrapolate(e, lam.p, nd, stream);
         }
-        solve_A(rhs.p, lam.p);
+        solve_A(rhs.p, lam.p, kSolveAdjoint);
         sens(0, lam.p, st, stb, t);
         // lambda_t becomes lambda_{t+1} of the next (earlier) step
         double* oldest = lamh[kAdjHist - 1].p;
@@ -340,12 +340,18 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     if (store_all) {
         for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
     } else {
+        // The adjoint step at t reads states t and t-1 only.  The sweep runs t = T, T-1, ..., 1, so
+        // state t is the state t-1 of the previous adjoint step: it is kept in st_hi instead of being
+        // recomputed from state t-1 (which would cost one Newmark step per adjoint step).  The schedule
+        // then needs the forward sweep plus the binomial recomputation only: 379 Newmark steps for
+        // 200 steps and 20 slots, against 579 when every adjoint step redoes its forward step.
+        const double* later = st_last.p;  // state T
         Revolve rv;
         rv.c = this;
         rv.adjoint_at = [&](int t) {
-            // st_cur = state t-1; take one forward step into st_next = state t
-            newmark_step(st_cur.p, st_next.p, t);
-            adjoint_step(t, st_next.p, st_cur.p);
+            adjoint_step(t, later, st_cur.p);
+            std::swap(st_hi.p, st_cur.p);  // state t-1 becomes the later state of step t-1
+            later = st_hi.p;
         };
         rv.reverse(0, T, (int)slots.size() - 1);
     }
