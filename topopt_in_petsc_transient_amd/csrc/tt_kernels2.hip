@@ -92,7 +92,7 @@ struct SensDev {
     double* df;
 };
 
-template <bool OBJ3>
+// (objective 3 runs in k_sens_mfma below; this loop serves the other objectives)
 __global__ __launch_bounds__(BS2) void k_sens(SensDev s, double* partials, unsigned* counter, double* out) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long nel = (long)s.g.ex * s.g.ey * s.g.ez;
@@ -148,20 +148,7 @@ __global__ __launch_bounds__(BS2) void k_sens(SensDev s, double* partials, unsig
         // objective terms of state t (EvaluateObjective / EvaluateObjectiveGradient, :1416-1514)
         double h0 = 0.0, dfx0 = 0.0;
         const double hdt = s.dt * 0.5;
-        if (OBJ3) {
-#pragma unroll
-            for (int q = 0; q < 24; q++) w[q] = s.u[3 * nd[q / 3] + q % 3];
-            double uKu = 0.0;
-#pragma unroll
-            for (int r = 0; r < 24; r++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
-                uKu = fma(w[r], acc, uKu);
-            }
-            h0 = hdt * (kf * uKu);
-            dfx0 = hdt * (dkf * uKu);
-        } else {
+        {
             double uKu = 0.0, vKv = 0.0, vMv = 0.0, uu = 0.0, vv = 0.0, aa = 0.0;
 #pragma unroll
             for (int q = 0; q < 24; q++) w[q] = s.u[3 * nd[q / 3] + q % 3];
@@ -208,6 +195,87 @@ __global__ __launch_bounds__(BS2) void k_sens(SensDev s, double* partials, unsig
     rfinish<1>(h, partials, counter, out);
 }
 
+// The same sensitivities with objective 3 on the matrix cores (v_mfma_f64_16x16x4f64).  A wave
+// takes batches of 16 consecutive elements, one per MFMA column, and forms per batch
+//     KE w_K, ME w_M and KE u   (each: 2 row tiles x 6 k-steps of 24 x 24 by 24 x 16).
+// Operand layout: lane j + 16 q holds the B value of k row q, i.e. dof 4 t + q of element j in
+// k-step t, and receives D rows 4 v + q of each row tile (measured on gfx950), i.e. dofs
+// 16 R + 4 v + q.  Both are the six dofs of element j that are = q (mod 4), so every lane
+// gathers just those six values of each input vector, and lambda^T KE w_K, lambda^T ME w_M and
+// u^T KE u are sums over the four lane groups (two shuffles, in a fixed order).  The element
+// matrices stay in registers as A fragments, so nothing is re-read per element.
+typedef double sens_d4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) {
+    const int lane = threadIdx.x & 63, j = lane & 15, q = lane >> 4, wave = threadIdx.x >> 6;
+    const long nel = (long)s.g.ex * s.g.ey * s.g.ez;
+    // A fragments: lane j + 16 q holds row j, k column q of each 16 x 4 block (rows 24..31 zero)
+    double aK[2][6], aM[2][6];
+#pragma unroll
+    for (int R = 0; R < 2; R++)
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+            const int r = 16 * R + j, c = 4 * t + q, rr = r < 24 ? r : 23;
+            const double k = s.KE[rr * 24 + c], m = s.M8[(rr / 3) * 8 + c / 3];
+            aK[R][t] = r < 24 ? k : 0.0;
+            aM[R][t] = (r < 24 && r % 3 == c % 3) ? m : 0.0;
+        }
+    const long sy = s.g.nx, sz = (long)s.g.nx * s.g.ny;
+    const double hdt = 0.5 * s.dt;
+    double h[1] = {0.0};
+#pragma unroll 1
+    for (int b = 0; b < 4; b++) {
+        const long e = (long)blockIdx.x * 256 + (wave * 4 + b) * 16 + j;
+        const bool valid = e < nel;
+        const long ec = valid ? e : nel - 1;
+        const long i = ec % s.g.ex, jj = (ec / s.g.ex) % s.g.ey, kk = ec / ((long)s.g.ex * s.g.ey);
+        const long n0 = i + sy * jj + sz * kk;
+        double lam[6], u[6], wk[6], wm[6];
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+            const int d = 4 * t + q, a = d / 3, c = d - 3 * a;  // element dof -> corner a, component c
+            const long node = n0 + ((a ^ (a >> 1)) & 1) + sy * ((a >> 1) & 1) + sz * (a >> 2);
+            const long x = 3 * node + c;
+            const double uu = s.u[x], x1 = s.s1[x], x2 = s.s2[x], x3 = s.s3[x];
+            lam[t] = s.lam[x];
+            u[t] = uu;
+            wk[t] = fma(s.wK[3], x3, fma(s.wK[2], x2, fma(s.wK[1], x1, s.wK[0] * uu)));
+            wm[t] = fma(s.wM[3], x3, fma(s.wM[2], x2, fma(s.wM[1], x1, s.wM[0] * uu)));
+        }
+        sens_d4 accK[2], accM[2], accU[2];
+#pragma unroll
+        for (int R = 0; R < 2; R++) accK[R] = accM[R] = accU[R] = sens_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 6; t++)
+#pragma unroll
+            for (int R = 0; R < 2; R++) {
+                accK[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], wk[t], accK[R], 0, 0, 0);
+                accM[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aM[R][t], wm[t], accM[R], 0, 0, 0);
+                accU[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], u[t], accU[R], 0, 0, 0);
+            }
+        // D row 4 v + q of row tile R is dof 4 (4 R + v) + q: the lane's value t = 4 R + v (t < 6)
+        double tK = 0.0, tM = 0.0, uKu = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+            const int R = t >> 2, v = t & 3;
+            tK = fma(accK[R][v], lam[t], tK);
+            tM = fma(accM[R][v], lam[t], tM);
+            uKu = fma(accU[R][v], u[t], uKu);
+        }
+        tK += __shfl_xor(tK, 16, 64);
+        tM += __shfl_xor(tM, 16, 64);
+        uKu += __shfl_xor(uKu, 16, 64);
+        tK += __shfl_xor(tK, 32, 64);
+        tM += __shfl_xor(tM, 32, 64);
+        uKu += __shfl_xor(uKu, 32, 64);
+        if (valid && q == 0) {
+            const double dkf = s.dkf[e], dmf = s.dmf[e], kf = s.kf[e], P = s.P[e];
+            s.df[e] += P * (hdt * (dkf * uKu)) + (dkf * tK + dmf * tM);
+            h[0] += (hdt * (kf * uKu)) * P;
+        }
+    }
+    rfinish<1>(h, partials, nullptr, nullptr);
+}
+
 void launch_sens(const SensArgs& a, Reducer& red, double* fx_out, hipStream_t st) {
     SensDev s;
     s.g = ga(a.g);
@@ -229,10 +297,11 @@ void launch_sens(const SensArgs& a, Reducer& red, double* fx_out, hipStream_t st
     // fx_out receives one partial objective value per block (nblk(nel) of them)
     const int nb = nblk(a.g.nel);
     (void)red;
+    static_assert(BS2 == 256, "k_sens_mfma: one block per 256 elements");
     if (a.objective == 3)
-        hipLaunchKernelGGL(k_sens<true>, dim3(nb), dim3(BS2), 0, st, s, fx_out, (unsigned*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_sens_mfma, dim3(nb), dim3(256), 0, st, s, fx_out);
     else
-        hipLaunchKernelGGL(k_sens<false>, dim3(nb), dim3(BS2), 0, st, s, fx_out, (unsigned*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_sens, dim3(nb), dim3(BS2), 0, st, s, fx_out, (unsigned*)nullptr, (double*)nullptr);
     TT_HIP(hipGetLastError());
 }
 
