@@ -177,6 +177,9 @@ void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void
 // r = b - A x
 void launch_residual(const OpLevel& L, double cK, double cM, const void* b, const void* x, void* r, bool fp64,
                      hipStream_t s);
+// r = b - A x (FP64) with the partials of b.b and r.r in two partial regions
+void launch_residual_dots(const OpLevel& L, double cK, double cM, const double* b, const double* x, double* r,
+                          Reducer& red, int bb_region, int rr_region, hipStream_t s);
 // Chebyshev step: r = b - A x; dn = c0*d + c1*dinv.r; xn = x + dn  (d may be null => c0 ignored)
 void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
                       const void* dinv, double c0, double c1, void* xn, void* dn, bool fp64, hipStream_t s);

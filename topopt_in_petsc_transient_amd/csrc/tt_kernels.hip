@@ -406,6 +406,36 @@ struct EpiResid {  // r = b - A x
     TT_EPI_CALL
 };
 
+// r = b - A x (FP64) with the partials of b.b and r.r, one per block into each of two regions
+struct EpiResidDots {
+    const double* b;
+    double* r;
+    double* pbb;
+    double* prr;
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st, const double (&)[3], double (&acc)[2]) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double ax = yK[c] + yM[c];
+            if ((cl >> c) & 1) ax = st.k(3 * n + c);
+            const double bi = b[3 * n + c], ri = bi - ax;
+            r[3 * n + c] = ri;
+            acc[0] = fma(bi, bi, acc[0]);
+            acc[1] = fma(ri, ri, acc[1]);
+        }
+    }
+    __device__ void done(double (&acc)[2]) const {
+        __shared__ double sh[16 * 2];
+        block_sum<2>(acc, sh);
+        if (threadIdx.x == 0) {
+            const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+            pbb[bid] = acc[0];
+            prr[bid] = acc[1];
+        }
+    }
+    TT_EPI_CALL
+};
+
 template <typename T>
 struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
     const T* b;
@@ -717,6 +747,15 @@ void launch_residual(const OpLevel& L, double cK, double cM, const void* b, cons
                                       EpiResid<float>{(const float*)b, (float*)r}, s);
 }
 
+void launch_residual_dots(const OpLevel& L, double cK, double cM, const double* b, const double* x, double* r,
+                          Reducer& red, int bb_region, int rr_region, hipStream_t s) {
+    TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
+    EpiResidDots e{b, r, red.part(bb_region), red.part(rr_region)};
+    const int nb = run_apply<double, false, true>(L, cK, cM, StageOne<double>{x}, e, s);
+    red.nblk[bb_region] = red.nblk[rr_region] = nb;
+    red.nv[bb_region] = red.nv[rr_region] = 1;
+}
+
 void launch_cheb_step(const OpLevel& L, double cK, double cM, const void* b, const void* x, const void* d,
                       const void* dinv, double c0, double c1, void* xn, void* dn, bool fp64, hipStream_t s) {
     if (fp64) {
@@ -792,11 +831,15 @@ void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, cons
     run_apply<double, true, true>(L, 1.0, 1.0, st, EpiRHS{F, fs, R}, s);
 }
 
-__global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first);
+// single-block reductions of partials: 1024 threads, so each sums ~5 of the ~5000 partials of a
+// fine-grid launch (latency bound: 256 threads took ~4.6 us per call)
+constexpr int BSR = 1024;
+__global__ __launch_bounds__(BSR) void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal,
+                                                  int first);
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
                       double* q, Reducer& red, bool first, hipStream_t s) {
     TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
-    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BS), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
+    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BSR), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
                        first ? 1 : 0);
     TT_HIP(hipGetLastError());
     StageFcg<double> st{z, p_old, red.scal.p, first ? 1 : 0, 0.0};
@@ -1255,7 +1298,7 @@ __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restric
 }
 // gamma = z.r and beta = (z.r - z.r_prev) / gamma_old (0 on a (re)start, and once the solve is
 // frozen, when gamma_old may be 0) from the partials of k_fcg_dots
-__global__ void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first) {
+__global__ __launch_bounds__(BSR) void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal, int first) {
     double d[2];
     sum_partials<2>(dots_part, dots_nblk, d);
     if (threadIdx.x == 0) {
@@ -1273,13 +1316,17 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
 }
 
 // Start (or restart) of a flexible-CG solve, from the partials of b.b and of r.r for the current
-// x: the tolerance (rtol ||b||)^2, whether b = 0, and whether x already meets the tolerance.
-__global__ void k_fcg_init(const double* __restrict__ bb_part, int bb_nblk, const double* __restrict__ rr_part,
+// x: the tolerance (rtol ||b||)^2, whether b = 0, and whether x already meets the tolerance.  The
+// b.b total replaces the first partial of its region (which then counts one block): the host reads
+// it back at every convergence look, and a copy of a few thousand partials takes a slower path
+// (measured: ~20 us more per look for 21 KB than for 8 KB).
+__global__ __launch_bounds__(BSR) void k_fcg_init(double* __restrict__ bb_part, int bb_nblk, const double* __restrict__ rr_part,
                            int rr_nblk, double rtol2, int reset_iters, double* scal) {
     double bb[1], rr[1];
     sum_partials<1>(bb_part, bb_nblk, bb);
     sum_partials<1>(rr_part, rr_nblk, rr);
     if (threadIdx.x == 0) {
+        bb_part[0] = bb[0];
         const double tol2 = rtol2 * bb[0];
         scal[SC_TOL2] = tol2;
         scal[SC_ZERO] = bb[0] == 0.0 ? 1.0 : 0.0;
@@ -1288,16 +1335,19 @@ __global__ void k_fcg_init(const double* __restrict__ bb_part, int bb_nblk, cons
     }
 }
 void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, bool reset_iters, hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(BS), 0, s, red.part(bb_region), red.nblk[bb_region],
-                       red.part(rr_region), red.nblk[rr_region], rtol2, reset_iters ? 1 : 0, red.scal.p);
+    hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(BSR), 0, s, red.part(bb_region), red.nblk[bb_region] * red.nv[bb_region],
+                       red.part(rr_region), red.nblk[rr_region] * red.nv[rr_region], rtol2, reset_iters ? 1 : 0,
+                       red.scal.p);
     TT_HIP(hipGetLastError());
+    red.nblk[bb_region] = 1;
+    red.nv[bb_region] = 1;
 }
 
 // alpha = gamma / (p.q) from the partials of the FCG apply (one block); gamma becomes gamma_old.
 // After the first iteration of a (re)start, the previous update's r.r (its partials) is checked
 // too: once it is <= (rtol ||b||)^2, SC_DONE stays set and alpha is 0 for the rest of the solve,
 // so x is frozen at the first iteration that met the tolerance.
-__global__ void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, const double* __restrict__ rr_part,
+__global__ __launch_bounds__(BSR) void k_fcg_alpha(const double* __restrict__ pq_part, int pq_nblk, const double* __restrict__ rr_part,
                             int rr_nblk, double* scal, int first) {
     double pq[1], rr[1] = {0.0};
     sum_partials<1>(pq_part, pq_nblk, pq);
@@ -1334,7 +1384,7 @@ __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ 
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        bool first, hipStream_t s) {
-    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(BS), 0, s, red.part(PB_PQ), red.nblk[PB_PQ], red.part(PB_RR),
+    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(BSR), 0, s, red.part(PB_PQ), red.nblk[PB_PQ], red.part(PB_RR),
                        red.nblk[PB_RR], red.scal.p, first ? 1 : 0);
     TT_HIP(hipGetLastError());
     const int nb = rb_blocks(n);

@@ -512,9 +512,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
     const double rtol = prm.rtol;
     // r = b - A x and the partials of b.b and r.r; the device derives the tolerance, the b = 0 case
     // and an already converged x from them, so the loop starts without a host round trip.
-    launch_residual(op, cK, cM, b, x, kr.p, true, stream);
-    launch_dot(b, b, nd, red, PB_MISC, stream);
-    launch_dot(kr.p, kr.p, nd, red, PB_MISC2, stream);
+    launch_residual_dots(op, cK, cM, b, x, kr.p, red, PB_MISC, PB_MISC2, stream);
     launch_fcg_init(red, PB_MISC, PB_MISC2, rtol * rtol, true, stream);
     // The residual is read back (a host round trip, ~25 us of idle GPU when the solve goes on) only
     // from one iteration before the fewest any of the last kHintLen solves of this kind took.  A look
@@ -598,7 +596,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
         it++;
         if (it < first_look) continue;
         double rr = 0.0, worked = 0.0;
-        reduce2(PB_RR, PB_MISC, rr, bb, &worked);  // PB_MISC holds the b.b partials for the whole solve
+        reduce2(PB_RR, PB_MISC, rr, bb, &worked);  // PB_MISC holds b.b (one value, set by k_fcg_init)
         moved = (int)worked;
         if (!std::isfinite(rr) || !std::isfinite(bb))
             throw Error{TT_ERR_NONFINITE, "non-finite right-hand side or residual in flexible CG"};
