@@ -160,9 +160,6 @@ struct Context {
     SolveResult solve_M(const double* b, double* x);                  // mass operator N M N + I - N, Jacobi FCG
     SolveResult fcg(double cK, double cM, bool use_mg, SolveKind kind, const double* b, double* x);
     int fcg_hist[kSolveKinds][kHintLen] = {};  // iteration counts of the last solves of each kind (0: none)
-    // sums of two single-value partial regions with one host round trip; after1 (optional) receives
-    // the value stored right after region r1's partials
-    void reduce2(int r1, int r2, double& a, double& b, double* after1 = nullptr);
     void sync();
     double read_scal(int slot);
     double reduce1(int region);  // host sum of a partial region (synchronises)

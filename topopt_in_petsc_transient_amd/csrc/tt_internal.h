@@ -103,6 +103,12 @@ struct Reducer {
     DBuf<double> scal;       // device scalars (see SC_* indices)
     double* host = nullptr;  // pinned mirror of scal
     double* hpart = nullptr; // pinned host copy of one partial region
+    // Host memory the kernels store into directly (pinned, mapped, coherent), so that the flexible-CG
+    // convergence look needs no copy: [0, nb) the r.r partials of the last k_fcg_update, [nb] the
+    // iterations that moved x; [kLookBB] the b.b of the solve (k_fcg_init).
+    static constexpr int kLookBB = 1100;
+    double* look = nullptr;
+    double* look_dev = nullptr;  // the same memory as the kernels address it
     int maxblocks = 0;
     int nblk[PB_COUNT] = {};  // blocks that wrote each region
     int nv[PB_COUNT] = {};

@@ -131,6 +131,9 @@ void Reducer::init(int maxb) {
     TT_HIP(hipMemset(scal.p, 0, sizeof(double) * SC_COUNT));
     TT_HIP(hipHostMalloc((void**)&host, sizeof(double) * SC_COUNT, hipHostMallocDefault));
     TT_HIP(hipHostMalloc((void**)&hpart, sizeof(double) * maxb * 4, hipHostMallocDefault));
+    TT_HIP(hipHostMalloc((void**)&look, sizeof(double) * (kLookBB + 1), hipHostMallocMapped | hipHostMallocCoherent));
+    TT_HIP(hipHostGetDevicePointer((void**)&look_dev, look, 0));
+    for (int i = 0; i <= kLookBB; i++) look[i] = 0.0;
 }
 void Reducer::host_sum(int region, hipStream_t s, double* out) {
     const int nb = nblk[region], v = nv[region];
@@ -143,6 +146,7 @@ void Reducer::host_sum(int region, hipStream_t s, double* out) {
 Reducer::~Reducer() {
     if (host) (void)hipHostFree(host);
     if (hpart) (void)hipHostFree(hpart);
+    if (look) (void)hipHostFree(look);
 }
 
 static inline int nblocks(long n, int per = BS) { return (int)((n + per - 1) / per); }
@@ -1321,12 +1325,13 @@ void launch_fcg_dots(const double* z, const double* r, const double* rprev, long
 // it back at every convergence look, and a copy of a few thousand partials takes a slower path
 // (measured: ~20 us more per look for 21 KB than for 8 KB).
 __global__ __launch_bounds__(BSR) void k_fcg_init(double* __restrict__ bb_part, int bb_nblk, const double* __restrict__ rr_part,
-                           int rr_nblk, double rtol2, int reset_iters, double* scal) {
+                           int rr_nblk, double rtol2, int reset_iters, double* scal, double* look) {
     double bb[1], rr[1];
     sum_partials<1>(bb_part, bb_nblk, bb);
     sum_partials<1>(rr_part, rr_nblk, rr);
     if (threadIdx.x == 0) {
         bb_part[0] = bb[0];
+        look[Reducer::kLookBB] = bb[0];
         const double tol2 = rtol2 * bb[0];
         scal[SC_TOL2] = tol2;
         scal[SC_ZERO] = bb[0] == 0.0 ? 1.0 : 0.0;
@@ -1337,7 +1342,7 @@ __global__ __launch_bounds__(BSR) void k_fcg_init(double* __restrict__ bb_part, 
 void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, bool reset_iters, hipStream_t s) {
     hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(BSR), 0, s, red.part(bb_region), red.nblk[bb_region] * red.nv[bb_region],
                        red.part(rr_region), red.nblk[rr_region] * red.nv[rr_region], rtol2, reset_iters ? 1 : 0,
-                       red.scal.p);
+                       red.scal.p, red.look_dev);
     TT_HIP(hipGetLastError());
     red.nblk[bb_region] = 1;
     red.nv[bb_region] = 1;
@@ -1366,7 +1371,7 @@ __global__ __launch_bounds__(BSR) void k_fcg_alpha(const double* __restrict__ pq
 // x and r as they are.
 __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ p, double* __restrict__ r,
                              double* __restrict__ rp, const double* __restrict__ q, long n,
-                             const double* __restrict__ scal, double* part) {
+                             const double* __restrict__ scal, double* part, double* look) {
     const double al = scal[SC_ALPHA];
     const bool zero = scal[SC_ZERO] != 0.0, done = scal[SC_DONE] != 0.0;
     double acc[1] = {0.0};
@@ -1379,8 +1384,13 @@ __global__ void k_fcg_update(double* __restrict__ x, const double* __restrict__ 
         r[i] = rn;
         acc[0] = fma(rn, rn, acc[0]);
     }
-    write_partials<1>(acc, part);
-    if (blockIdx.x == 0 && threadIdx.x == 0) part[gridDim.x] = scal[SC_ITERS];  // read with the partials
+    __shared__ double sh[16];
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) {  // the partial for k_fcg_alpha, and its copy for the host's look
+        part[blockIdx.x] = acc[0];
+        look[blockIdx.x] = acc[0];
+        if (blockIdx.x == 0) look[gridDim.x] = scal[SC_ITERS];
+    }
 }
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        bool first, hipStream_t s) {
@@ -1388,7 +1398,9 @@ void launch_fcg_update(double* x, const double* p, double* r, double* rprev, con
                        red.nblk[PB_RR], red.scal.p, first ? 1 : 0);
     TT_HIP(hipGetLastError());
     const int nb = rb_blocks(n);
-    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.scal.p, red.part(PB_RR));
+    TT_CHECK(nb < Reducer::kLookBB, -3, "look buffer too small");
+    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(BS), 0, s, x, p, r, rprev, q, n, red.scal.p, red.part(PB_RR),
+                       red.look_dev);
     red.nblk[PB_RR] = nb;
     red.nv[PB_RR] = 1;
     TT_HIP(hipGetLastError());

@@ -471,19 +471,6 @@ void Multigrid::apply(const double* r, double* z) {
 // ------------------------------------------------------------------ context-level Krylov
 void Context::sync() { TT_HIP(hipStreamSynchronize(stream)); }
 
-void Context::reduce2(int r1, int r2, double& a, double& b, double* after1) {
-    const int n1 = red.nblk[r1] * red.nv[r1], n2 = red.nblk[r2] * red.nv[r2];
-    const int c1 = n1 + (after1 ? 1 : 0);
-    TT_HIP(hipMemcpyAsync(red.hpart, red.part(r1), sizeof(double) * c1, hipMemcpyDeviceToHost, stream));
-    TT_HIP(hipMemcpyAsync(red.hpart + c1, red.part(r2), sizeof(double) * n2, hipMemcpyDeviceToHost, stream));
-    TT_HIP(hipStreamSynchronize(stream));
-    a = 0.0;
-    b = 0.0;
-    for (int i = 0; i < n1; i++) a += red.hpart[i];
-    for (int i = 0; i < n2; i++) b += red.hpart[c1 + i];
-    if (after1) *after1 = red.hpart[n1];
-}
-
 double Context::read_scal(int slot) {
     TT_HIP(hipMemcpyAsync(red.host + slot, red.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, stream));
     TT_HIP(hipStreamSynchronize(stream));
@@ -595,9 +582,17 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
         first = false;
         it++;
         if (it < first_look) continue;
-        double rr = 0.0, worked = 0.0;
-        reduce2(PB_RR, PB_MISC, rr, bb, &worked);  // PB_MISC holds b.b (one value, set by k_fcg_init)
-        moved = (int)worked;
+        // r.r partials, the device's count of iterations that moved x and b.b, as the kernels stored
+        // them into mapped host memory: no copy, just the wait
+        double rr = 0.0;
+        sync();
+        {
+            const volatile double* lk = red.look;
+            const int nb = red.nblk[PB_RR];
+            for (int i = 0; i < nb; i++) rr += lk[i];
+            moved = (int)lk[nb];
+            bb = lk[Reducer::kLookBB];
+        }
         if (!std::isfinite(rr) || !std::isfinite(bb))
             throw Error{TT_ERR_NONFINITE, "non-finite right-hand side or residual in flexible CG"};
         if (bb == 0.0) {  // b = 0: the device has set x = 0
