@@ -117,7 +117,10 @@ static int revolve_right(int n, int s) {
 
 struct Revolve {
     Context* c;
-    std::function<void(int)> adjoint_at;  // adjoint step t; the working state st_cur holds state t-1
+    // adjoint step t; state t-1 is at prev (a checkpoint slot or st_cur), read only
+    std::function<void(int, const double*)> adjoint_at;
+    // called before the slot of a checkpointed step is freed (the caller may still point into it)
+    std::function<void(int)> on_release;
     int find_slot(int step) const {
         for (size_t i = 0; i < c->slot_step.size(); i++)
             if (c->slot_step[i] == step) return (int)i;
@@ -128,53 +131,54 @@ struct Revolve {
             if (c->slot_step[i] < 0) return (int)i;
         return -1;
     }
-    void restore(int step) {
+    const double* slot_of(int step) const {
         const int k = find_slot(step);
         TT_CHECK(k >= 0, TT_ERR_INTERNAL, "revolve: checkpoint missing");
-        launch_copy(c->slots[k].p, c->st_cur.p, Context::kStateVecs * c->g.nd, true, true, c->stream);
+        return c->slots[k].p;
     }
-    void advance(int from, int to) {  // st_cur: state from -> state to
+    // states from+1 .. to, starting from the checkpoint of `from` read in place (no restore copy);
+    // the last one goes straight into `last` (a free checkpoint slot) when given, else it ends in
+    // st_cur.  Returns where state `to` is.
+    const double* advance(int from, int to, double* last) {
+        const double* src = slot_of(from);
         for (int t = from + 1; t <= to; t++) {
-            c->newmark_step(c->st_cur.p, c->st_next.p, t);
+            if (t == to && last) {
+                c->newmark_step(src, last, t);
+                return last;
+            }
+            c->newmark_step(src, c->st_next.p, t);
             std::swap(c->st_cur.p, c->st_next.p);
+            src = c->st_cur.p;
         }
-    }
-    void shot(int step) {
-        int k = find_slot(step);
-        if (k >= 0) return;
-        k = free_slot();
-        TT_CHECK(k >= 0, TT_ERR_INTERNAL, "revolve: no free checkpoint slot");
-        launch_copy(c->st_cur.p, c->slots[k].p, Context::kStateVecs * c->g.nd, true, true, c->stream);
-        c->slot_step[k] = step;
+        return src;
     }
     void release(int step) {
         const int k = find_slot(step);
-        if (k > 0) c->slot_step[k] = -1;
+        if (k > 0) {
+            if (on_release) on_release(step);
+            c->slot_step[k] = -1;
+        }
     }
     // reverse steps r, r-1, ..., l+1; state l is checkpointed; s free slots
     void reverse(int l, int r, int s) {
         if (r - l == 1) {
-            restore(l);
-            adjoint_at(r);
+            adjoint_at(r, slot_of(l));
             return;
         }
         if (s == 0) {
-            for (int t = r; t > l; t--) {
-                restore(l);
-                advance(l, t - 1);
-                adjoint_at(t);
-            }
+            for (int t = r; t > l; t--) adjoint_at(t, t - 1 == l ? slot_of(l) : advance(l, t - 1, nullptr));
             return;
         }
         const int m = r - revolve_right(r - l, s);
         if (find_slot(m) < 0) {
-            // nearest stored state at or below m
+            // from the nearest stored state at or below m, the last step written into a free slot
             int best = l;
             for (size_t i = 0; i < c->slot_step.size(); i++)
                 if (c->slot_step[i] > best && c->slot_step[i] <= m) best = c->slot_step[i];
-            restore(best);
-            advance(best, m);
-            shot(m);
+            const int k = free_slot();
+            TT_CHECK(k >= 0, TT_ERR_INTERNAL, "revolve: no free checkpoint slot");
+            advance(best, m, c->slots[k].p);
+            c->slot_step[k] = m;
         }
         reverse(m, r, s - 1);
         release(m);
@@ -341,17 +345,31 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
         for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
     } else {
         // The adjoint step at t reads states t and t-1 only.  The sweep runs t = T, T-1, ..., 1, so
-        // state t is the state t-1 of the previous adjoint step: it is kept in st_hi instead of being
-        // recomputed from state t-1 (which would cost one Newmark step per adjoint step).  The schedule
-        // then needs the forward sweep plus the binomial recomputation only: 379 Newmark steps for
-        // 200 steps and 20 slots, against 579 when every adjoint step redoes its forward step.
+        // state t is the state t-1 of the previous adjoint step: it is kept (in its checkpoint slot,
+        // or moved to st_hi) instead of being recomputed from state t-1, which would cost one Newmark
+        // step per adjoint step.  The schedule then needs the forward sweep plus the binomial
+        // recomputation only: 379 Newmark steps for 200 steps and 20 slots, against 579 when every
+        // adjoint step redoes its forward step.  Checkpoints are read in place, never restored.
         const double* later = st_last.p;  // state T
+        int later_step = -1;              // the checkpointed step `later` points into, or -1
         Revolve rv;
         rv.c = this;
-        rv.adjoint_at = [&](int t) {
-            adjoint_step(t, later, st_cur.p);
-            std::swap(st_hi.p, st_cur.p);  // state t-1 becomes the later state of step t-1
+        rv.adjoint_at = [&](int t, const double* prev) {
+            adjoint_step(t, later, prev);
+            if (prev == st_cur.p) {  // out of the way of the next recomputation
+                std::swap(st_hi.p, st_cur.p);
+                later = st_hi.p;
+                later_step = -1;
+            } else {
+                later = prev;  // a checkpoint slot, valid until it is released
+                later_step = t - 1;
+            }
+        };
+        rv.on_release = [&](int step) {
+            if (step != later_step) return;
+            launch_copy(later, st_hi.p, 3 * nd, true, true, stream);  // u, v, a: what an adjoint step reads
             later = st_hi.p;
+            later_step = -1;
         };
         rv.reverse(0, T, (int)slots.size() - 1);
     }
