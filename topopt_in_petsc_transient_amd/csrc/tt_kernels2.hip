@@ -92,120 +92,25 @@ struct SensDev {
     double* df;
 };
 
-// (objective 3 runs in k_sens_mfma below; this loop serves the other objectives)
-__global__ __launch_bounds__(BS2) void k_sens(SensDev s, double* partials, unsigned* counter, double* out) {
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const long nel = (long)s.g.ex * s.g.ey * s.g.ez;
-    double h[1] = {0.0};
-    if (e < nel) {
-        long nd[8];
-        elem_nodes(s.g, e, nd);
-        double w[24];
-        // w_K
-#pragma unroll
-        for (int q = 0; q < 24; q++) {
-            const long d = 3 * nd[q / 3] + q % 3;
-            double x = s.wK[0] * s.u[d];
-            if (s.wK[1] != 0.0) x = fma(s.wK[1], s.s1[d], x);
-            if (s.wK[2] != 0.0) x = fma(s.wK[2], s.s2[d], x);
-            if (s.wK[3] != 0.0) x = fma(s.wK[3], s.s3[d], x);
-            w[q] = x;
-        }
-        double tK = 0.0;
-#pragma unroll
-        for (int r = 0; r < 24; r++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
-            tK = fma(s.lam[3 * nd[r / 3] + r % 3], acc, tK);
-        }
-        // w_M (mass is M8 (x) I3)
-#pragma unroll
-        for (int q = 0; q < 24; q++) {
-            const long d = 3 * nd[q / 3] + q % 3;
-            double x = s.wM[0] * s.u[d];
-            if (s.wM[1] != 0.0) x = fma(s.wM[1], s.s1[d], x);
-            if (s.wM[2] != 0.0) x = fma(s.wM[2], s.s2[d], x);
-            if (s.wM[3] != 0.0) x = fma(s.wM[3], s.s3[d], x);
-            w[q] = x;
-        }
-        double tM = 0.0;
-#pragma unroll
-        for (int ai = 0; ai < 8; ai++) {
-            double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const double mab = s.M8[ai * 8 + b];
-                m0 = fma(mab, w[3 * b], m0);
-                m1 = fma(mab, w[3 * b + 1], m1);
-                m2 = fma(mab, w[3 * b + 2], m2);
-            }
-            const long d = 3 * nd[ai];
-            tM = fma(s.lam[d], m0, fma(s.lam[d + 1], m1, fma(s.lam[d + 2], m2, tM)));
-        }
-        const double dkf = s.dkf[e], dmf = s.dmf[e], kf = s.kf[e], mf = s.mf[e], P = s.P[e];
-        const double t3 = dkf * tK + dmf * tM;
-        // objective terms of state t (EvaluateObjective / EvaluateObjectiveGradient, :1416-1514)
-        double h0 = 0.0, dfx0 = 0.0;
-        const double hdt = s.dt * 0.5;
-        {
-            double uKu = 0.0, vKv = 0.0, vMv = 0.0, uu = 0.0, vv = 0.0, aa = 0.0;
-#pragma unroll
-            for (int q = 0; q < 24; q++) w[q] = s.u[3 * nd[q / 3] + q % 3];
-            for (int r = 0; r < 24; r++) {
-                double acc = 0.0;
-                for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
-                uKu = fma(w[r], acc, uKu);
-                uu = fma(w[r], w[r], uu);
-            }
-#pragma unroll
-            for (int q = 0; q < 24; q++) w[q] = s.v[3 * nd[q / 3] + q % 3];
-            for (int r = 0; r < 24; r++) {
-                double acc = 0.0;
-                for (int c = 0; c < 24; c++) acc = fma(s.KE[r * 24 + c], w[c], acc);
-                vKv = fma(w[r], acc, vKv);
-                vv = fma(w[r], w[r], vv);
-            }
-            for (int ai = 0; ai < 8; ai++)
-                for (int b = 0; b < 8; b++)
-                    for (int c = 0; c < 3; c++) vMv = fma(w[3 * ai + c] * s.M8[ai * 8 + b], w[3 * b + c], vMv);
-            for (int q = 0; q < 24; q++) {
-                const double x = s.a[3 * nd[q / 3] + q % 3];
-                aa = fma(x, x, aa);
-            }
-            switch (s.objective) {
-            case 0: h0 = hdt * uu; break;
-            case 1: h0 = hdt * vv; break;
-            case 2: h0 = hdt * aa; break;
-            case 3: h0 = hdt * (kf * uKu); dfx0 = hdt * (dkf * uKu); break;
-            case 4: h0 = hdt * (mf * vMv); dfx0 = hdt * (dmf * vMv); break;
-            case 5:
-                h0 = hdt * (kf * uKu + mf * vMv);
-                dfx0 = hdt * (dkf * uKu + dmf * vMv);
-                break;
-            default:
-                h0 = hdt * (s.al * mf * vMv + s.be * kf * vKv);
-                dfx0 = hdt * (s.al * dmf * vMv + s.be * dkf * vKv);
-                break;
-            }
-        }
-        s.df[e] += P * dfx0 + t3;
-        h[0] = h0 * P;
-    }
-    rfinish<1>(h, partials, counter, out);
-}
-
-// The same sensitivities with objective 3 on the matrix cores (v_mfma_f64_16x16x4f64).  A wave
-// takes batches of 16 consecutive elements, one per MFMA column, and forms per batch
-//     KE w_K, ME w_M and KE u   (each: 2 row tiles x 6 k-steps of 24 x 24 by 24 x 16).
+// On the matrix cores (v_mfma_f64_16x16x4f64).  A wave takes batches of 16
+// consecutive elements, one per MFMA column, and forms per batch
+//     KE w_K and ME w_M, plus the objective's element products: KE u (objectives 3, 5), ME v (4-6),
+//     KE v (6)   (each: 2 row tiles x 6 k-steps of 24 x 24 by 24 x 16).
 // Operand layout: lane j + 16 q holds the B value of k row q, i.e. dof 4 t + q of element j in
 // k-step t, and receives D rows 4 v + q of each row tile (measured on gfx950), i.e. dofs
 // 16 R + 4 v + q.  Both are the six dofs of element j that are = q (mod 4), so every lane
-// gathers just those six values of each input vector, and lambda^T KE w_K, lambda^T ME w_M and
-// u^T KE u are sums over the four lane groups (two shuffles, in a fixed order).  The element
-// matrices stay in registers as A fragments, so nothing is re-read per element.
+// gathers just those six values of each input vector, and lambda^T KE w_K, lambda^T ME w_M and the
+// objective's quadratic forms (u^T u, v^T v, a^T a for objectives 0-2) are sums over the four lane
+// groups (two shuffles, in a fixed order).  The element matrices stay in registers as A fragments.
 typedef double sens_d4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double lane_group_sum(double x) {
+    x += __shfl_xor(x, 16, 64);
+    return x + __shfl_xor(x, 32, 64);
+}
+template <int OBJ>
 __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) {
+    constexpr bool KU = OBJ == 3 || OBJ == 5, MV = OBJ >= 4, KV = OBJ == 6;
+    constexpr bool LV = OBJ == 1 || OBJ >= 4, LA = OBJ == 2;  // v or a of state t needed
     const int lane = threadIdx.x & 63, j = lane & 15, q = lane >> 4, wave = threadIdx.x >> 6;
     const long nel = (long)s.g.ex * s.g.ey * s.g.ez;
     // A fragments: lane j + 16 q holds row j, k column q of each 16 x 4 block (rows 24..31 zero)
@@ -229,7 +134,7 @@ __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) 
         const long ec = valid ? e : nel - 1;
         const long i = ec % s.g.ex, jj = (ec / s.g.ex) % s.g.ey, kk = ec / ((long)s.g.ex * s.g.ey);
         const long n0 = i + sy * jj + sz * kk;
-        double lam[6], u[6], wk[6], wm[6];
+        double lam[6], u[6], wk[6], wm[6], w2[6];
 #pragma unroll
         for (int t = 0; t < 6; t++) {
             const int d = 4 * t + q, a = d / 3, c = d - 3 * a;  // element dof -> corner a, component c
@@ -240,37 +145,53 @@ __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) 
             u[t] = uu;
             wk[t] = fma(s.wK[3], x3, fma(s.wK[2], x2, fma(s.wK[1], x1, s.wK[0] * uu)));
             wm[t] = fma(s.wM[3], x3, fma(s.wM[2], x2, fma(s.wM[1], x1, s.wM[0] * uu)));
+            w2[t] = LV ? s.v[x] : LA ? s.a[x] : 0.0;  // the state-t vector of the objective
         }
-        sens_d4 accK[2], accM[2], accU[2];
+        sens_d4 accK[2], accM[2], accU[2], accVM[2], accVK[2];
 #pragma unroll
-        for (int R = 0; R < 2; R++) accK[R] = accM[R] = accU[R] = sens_d4{0.0, 0.0, 0.0, 0.0};
+        for (int R = 0; R < 2; R++) accK[R] = accM[R] = accU[R] = accVM[R] = accVK[R] = sens_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < 6; t++)
 #pragma unroll
             for (int R = 0; R < 2; R++) {
                 accK[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], wk[t], accK[R], 0, 0, 0);
                 accM[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aM[R][t], wm[t], accM[R], 0, 0, 0);
-                accU[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], u[t], accU[R], 0, 0, 0);
+                if (KU) accU[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], u[t], accU[R], 0, 0, 0);
+                if (MV) accVM[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aM[R][t], w2[t], accVM[R], 0, 0, 0);
+                if (KV) accVK[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], w2[t], accVK[R], 0, 0, 0);
             }
         // D row 4 v + q of row tile R is dof 4 (4 R + v) + q: the lane's value t = 4 R + v (t < 6)
-        double tK = 0.0, tM = 0.0, uKu = 0.0;
+        double tK = 0.0, tM = 0.0, fU = 0.0, fVM = 0.0, fVK = 0.0;
 #pragma unroll
         for (int t = 0; t < 6; t++) {
             const int R = t >> 2, v = t & 3;
             tK = fma(accK[R][v], lam[t], tK);
             tM = fma(accM[R][v], lam[t], tM);
-            uKu = fma(accU[R][v], u[t], uKu);
+            if (KU) fU = fma(accU[R][v], u[t], fU);
+            if (OBJ == 0) fU = fma(u[t], u[t], fU);
+            if (OBJ == 1 || OBJ == 2) fU = fma(w2[t], w2[t], fU);
+            if (MV) fVM = fma(accVM[R][v], w2[t], fVM);
+            if (KV) fVK = fma(accVK[R][v], w2[t], fVK);
         }
-        tK += __shfl_xor(tK, 16, 64);
-        tM += __shfl_xor(tM, 16, 64);
-        uKu += __shfl_xor(uKu, 16, 64);
-        tK += __shfl_xor(tK, 32, 64);
-        tM += __shfl_xor(tM, 32, 64);
-        uKu += __shfl_xor(uKu, 32, 64);
+        tK = lane_group_sum(tK);
+        tM = lane_group_sum(tM);
+        fU = lane_group_sum(fU);
+        if (MV) fVM = lane_group_sum(fVM);
+        if (KV) fVK = lane_group_sum(fVK);
         if (valid && q == 0) {
-            const double dkf = s.dkf[e], dmf = s.dmf[e], kf = s.kf[e], P = s.P[e];
-            s.df[e] += P * (hdt * (dkf * uKu)) + (dkf * tK + dmf * tM);
-            h[0] += (hdt * (kf * uKu)) * P;
+            const double dkf = s.dkf[e], dmf = s.dmf[e], kf = s.kf[e], mf = s.mf[e], P = s.P[e];
+            // objective terms of state t (EvaluateObjective / EvaluateObjectiveGradient, :1416-1514)
+            double h0 = 0.0, dfx0 = 0.0;
+            if (OBJ <= 2) h0 = hdt * fU;
+            if (OBJ == 3) { h0 = hdt * (kf * fU); dfx0 = hdt * (dkf * fU); }
+            if (OBJ == 4) { h0 = hdt * (mf * fVM); dfx0 = hdt * (dmf * fVM); }
+            if (OBJ == 5) { h0 = hdt * (kf * fU + mf * fVM); dfx0 = hdt * (dkf * fU + dmf * fVM); }
+            if (OBJ == 6) {
+                h0 = hdt * (s.al * mf * fVM + s.be * kf * fVK);
+                dfx0 = hdt * (s.al * dmf * fVM + s.be * dkf * fVK);
+            }
+            s.df[e] += P * dfx0 + (dkf * tK + dmf * tM);
+            h[0] += h0 * P;
         }
     }
     rfinish<1>(h, partials, nullptr, nullptr);
@@ -298,10 +219,15 @@ void launch_sens(const SensArgs& a, Reducer& red, double* fx_out, hipStream_t st
     const int nb = nblk(a.g.nel);
     (void)red;
     static_assert(BS2 == 256, "k_sens_mfma: one block per 256 elements");
-    if (a.objective == 3)
-        hipLaunchKernelGGL(k_sens_mfma, dim3(nb), dim3(256), 0, st, s, fx_out);
-    else
-        hipLaunchKernelGGL(k_sens, dim3(nb), dim3(BS2), 0, st, s, fx_out, (unsigned*)nullptr, (double*)nullptr);
+    switch (a.objective) {
+    case 0: hipLaunchKernelGGL(k_sens_mfma<0>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    case 1: hipLaunchKernelGGL(k_sens_mfma<1>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    case 2: hipLaunchKernelGGL(k_sens_mfma<2>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    case 3: hipLaunchKernelGGL(k_sens_mfma<3>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    case 4: hipLaunchKernelGGL(k_sens_mfma<4>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    case 5: hipLaunchKernelGGL(k_sens_mfma<5>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    default: hipLaunchKernelGGL(k_sens_mfma<6>, dim3(nb), dim3(256), 0, st, s, fx_out); break;
+    }
     TT_HIP(hipGetLastError());
 }
 
