@@ -42,7 +42,6 @@ struct Multigrid {
     int nc = 0;
     bool dense = true;
     DBuf<double> Ainv, AinvP, sprow, spcol, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest level
-    DBuf<float> AinvP32;  // the packed inverse rounded to FP32, for an FP32 hierarchy
     DBuf<int> dinfo;
     void* blas = nullptr;  // rocblas_handle used by the rocSOLVER factorisation of the coarsest operator
     int coarse_deg = 12;

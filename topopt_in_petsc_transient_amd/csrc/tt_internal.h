@@ -170,7 +170,6 @@ void tile_bc_flags(const Grid& g, const std::vector<uint8_t>& mask, int tz, std:
 long sym_packed_size(int n);
 void launch_sym_pack(const double* A, int n, double* P, hipStream_t s);
 void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
-void launch_sym_solve(const float* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
 // Column-major lower triangle of an n x n matrix copied into its upper triangle.
 void launch_mirror_lower(double* A, int n, hipStream_t s);
 constexpr int AF32_WORDS = 18 * 64 + 3 * 2 * 64 * 4;  // FP32 fragments followed by the FP16-split ones

@@ -1189,7 +1189,7 @@ void launch_sym_pack(const double* A, int n, double* P, hipStream_t s) {
 }
 // per tile: row sums A_IJ x_J and (off the diagonal) column sums A_IJ^T x_I, written as FP64
 // partials.  Each thread first loads its 16 tile values (16-byte loads, all in flight), then the
-// tile goes through LDS once.  PT = double, or float for the FP32 preconditioner (half the bytes).
+// tile goes through LDS once.
 template <typename PT>
 __global__ __launch_bounds__(256) void k_sym_tiles(const PT* __restrict__ P, const double* __restrict__ x, int n,
                                                    double* __restrict__ prow, double* __restrict__ pcol) {
@@ -1259,10 +1259,6 @@ static void sym_solve(const PT* P, const double* b, double* x, int n, double* pr
     TT_HIP(hipGetLastError());
 }
 void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol,
-                      hipStream_t s) {
-    sym_solve(P, b, x, n, prow, pcol, s);
-}
-void launch_sym_solve(const float* P, const double* b, double* x, int n, double* prow, double* pcol,
                       hipStream_t s) {
     sym_solve(P, b, x, n, prow, pcol, s);
 }

@@ -127,7 +127,6 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
     if (dense) {
         Ainv.alloc((size_t)nc * nc);
         AinvP.alloc(sym_packed_size(nc));
-        if (!L.back()->fp64) AinvP32.alloc(sym_packed_size(nc));
         {
             const long nt = (nc + 63) / 64, ntiles = nt * (nt + 1) / 2;
             sprow.alloc(ntiles * 64);
@@ -272,7 +271,6 @@ void Multigrid::assemble_coarse_inverse() {
     TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
     launch_mirror_lower(Ainv.p, n, s);
     launch_sym_pack(Ainv.p, n, AinvP.p, s);
-    if (AinvP32.p) launch_copy(AinvP.p, AinvP32.p, sym_packed_size(n), true, false, s);
     TT_HIP(hipStreamSynchronize(s));
 }
 
@@ -335,8 +333,7 @@ OpLevel Multigrid::coarse_op64() const {
 }
 
 // direct64: the FP32 level above restricted straight into cb and prolongs straight from cx.
-// An FP32 hierarchy applies the inverse rounded to FP32 (half the bytes; the products and their
-// sums stay FP64): measured on the 128x64x64 Newmark sweeps, the Krylov iteration counts do not change.
+// The coarsest solve is FP64 in both precision modes.
 void Multigrid::coarse_solve(bool direct64) {
     MGLevel& lc = *L.back();
     hipStream_t s = ctx->stream;
@@ -345,10 +342,8 @@ void Multigrid::coarse_solve(bool direct64) {
     if (!lc.fp64) {
         b64 = cb.p;
         x64 = cx.p;
-        if (dense) {
-            if (!direct64) launch_copy(lc.b.p, cb.p, nc, false, true, s);
-            launch_sym_solve(AinvP32.p, b64, x64, nc, sprow.p, spcol.p, s);
-            if (!direct64) launch_copy(x64, lc.x.p, nc, true, false, s);
+        if (direct64) {
+            launch_sym_solve(AinvP.p, b64, x64, nc, sprow.p, spcol.p, s);
             return;
         }
         launch_copy(lc.b.p, cb.p, nc, false, true, s);
