@@ -166,8 +166,13 @@ struct GridArgs {
 };
 static inline GridArgs gargs(const Grid& g) { return GridArgs{g.nx, g.ny, g.nz, g.ex, g.ey, g.ez}; }
 
-// Per-node contraction: yK = sum_e kfe * KE[a rows] u_e, yM = sum_e mfe * (M8 (x) I3)[a rows] w_e
-template <typename T, bool TWO>
+// Per-node contraction: yK = sum_e kfe * KE[a rows] u_e, yM = sum_e mfe * (M8 (x) I3)[a rows] w_e.
+// FENCE keeps each element's matrix rows from being hoisted across the node's 8 elements (the
+// compiler otherwise holds the 576 + 64 uniform values in scalar registers and spills them): with
+// it, the two-input right-hand side compiles to 111 instead of 241 VGPRs and 12 instead of 108
+// spilled SGPRs.  It is applied where K and M results leave the kernel separately (two inputs, or
+// the split K / M outputs); with one summed output the same fence makes the code larger.
+template <typename T, bool TWO, bool FENCE>
 __device__ __forceinline__ void node_contract(const T* __restrict__ sU, const T* __restrict__ sW,
                                               const T* __restrict__ sk, const T* __restrict__ sm,
                                               const T* __restrict__ KE, const T* __restrict__ M8, int tx, int ty,
@@ -178,6 +183,7 @@ __device__ __forceinline__ void node_contract(const T* __restrict__ sU, const T*
         for (int ey = 0; ey < 2; ey++)
 #pragma unroll
             for (int ex = 0; ex < 2; ex++) {
+                if (FENCE) __asm__ __volatile__("" ::: "memory");
                 const int a = corner_id(1 - ex, 1 - ey, 1 - ez);
                 const int eidx = ((tz + ez) * EYs + (ty + ey)) * EXs + (tx + ex);
                 const T kfe = sk[eidx], mfe = sm[eidx];
@@ -250,6 +256,7 @@ struct StageThree {  // wK = sum_i cK_i in_i, wM = sum_i cM_i in_i (FP64)
     __device__ __forceinline__ double m(unsigned i) const { return m0 * ld(u, i) + m1 * ld(v, i) + m2 * ld(a, i); }
 };
 
+struct EpiSplit;
 template <typename T, typename CT, bool TWO, bool MASKIN, class Stage, class Epi>
 __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ KE, const T* __restrict__ M8,
                                               const CT* __restrict__ kf, const CT* __restrict__ mf, T cK, T cM,
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
     long n = 0;
     uint8_t cl = 0;
     if (inside) {
-        node_contract<T, TWO>(sU, sW, sk, sm, KE, M8, tx, ty, tz, yK, yM);
+        node_contract<T, TWO, TWO || std::is_same<Epi, EpiSplit>::value>(sU, sW, sk, sm, KE, M8, tx, ty, tz, yK, yM);
         n = gi + (long)g.nx * (gj + (long)g.ny * gk);
         cl = mask[n];
     }
