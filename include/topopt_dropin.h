@@ -12,8 +12,11 @@
  *
  * Differences a caller can observe:
  *   - Filter supports the PDE filter only (filterT = 2, Filter.cc:60-119 / 122-206);
- *   - MMA supports one constraint (m = 1) and no restart constructors;
- *   - WriteRestartFiles() writes nothing; the reduced-order path (reduction = TRUE) is rejected;
+ *   - MMA supports one constraint (m = 1);
+ *   - restart files (Restart0x.dat, Restart0x_itr_f0.dat, RestartSol0x.dat) are written and read in
+ *     PETSc's binary Vec format with the reference's options; the cp%04d.dat checkpoint files are
+ *     not written (the binomial checkpoints stay in device memory), and -solutionAsU0 is rejected;
+ *   - the reduced-order path (reduction = TRUE) is rejected;
  *   - numerical non-convergence and non-finite values are reported as errors (< 0).
  */
 #ifndef TOPOPT_DROPIN_H
@@ -128,6 +131,9 @@ class TopOpt {
    private:
     void Init(PetscInt nconstraint);
     PetscErrorCode SetUp();
+    // restart (TopOpt.cc:533-735): MMA state vectors and the two alternating file sets
+    Vec xo1, xo2, U, L;
+    std::string filename00, filename00Itr, filename01, filename01Itr;
 };
 
 /* LinearElasticity.h:29-85 (LinearElasticity.cc): transient elasticity physics. */
@@ -160,6 +166,9 @@ class LinearElasticity {
     PetscInt Ts;
     tt_ctx* ctx;
     bool forward_done;
+    // restart (LinearElasticity.cc:2457-2496, 2679-2723): two alternating RestartSol files
+    PetscBool restart, flip;
+    std::string filename00, filename01;
     Vec state[3];
     PetscErrorCode SetMaterial(PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin, PetscScalar Rmax,
                                PetscScalar penalK, PetscScalar penalM, PetscScalar dalpha, PetscScalar dbeta);
@@ -204,6 +213,10 @@ class MMA {
    public:
     MMA(PetscInt n, PetscInt m, Vec x);
     MMA(PetscInt n, PetscInt m, Vec x, PetscScalar* a, PetscScalar* c, PetscScalar* d);
+    /* restart constructors (MMA.cc:6-177): iteration k, previous iterates and asymptotes */
+    MMA(PetscInt n, PetscInt m, PetscInt k, Vec xo1, Vec xo2, Vec U, Vec L, PetscScalar* a, PetscScalar* c,
+        PetscScalar* d);
+    MMA(PetscInt n, PetscInt m, PetscInt k, Vec xo1, Vec xo2, Vec U, Vec L);
     ~MMA();
     PetscErrorCode Update(Vec xval, Vec dfdx, PetscScalar* gx, Vec* dgdx, Vec xmin, Vec xmax);
     PetscErrorCode Restart(Vec xo1, Vec xo2, Vec U, Vec L);

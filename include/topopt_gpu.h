@@ -163,6 +163,31 @@ int tt_mma_kkt_residual(tt_mma* mma, const double* x, const double* dfdx, const 
                         const double* xmin, const double* xmax, double* norm2, double* normInf,
                         int on_device);                                                  /* MMA.cc:416-484 */
 int tt_mma_get_asymptotes(tt_mma* mma, double* L, double* U, double* lam); /* host arrays (nel, nel, 1) */
+/* MMA restart state: iteration counter k, the two previous iterates and the asymptotes.  get replaces
+   MMA::Restart (MMA.cc:307-350), set the restart constructor MMA(n, m, k, xo1, xo2, U, L, a, c, d)
+   (MMA.cc:6-90).  Host arrays of nel values. */
+int tt_mma_get_state(tt_mma* mma, int* k, double* xo1, double* xo2, double* U, double* L);
+int tt_mma_set_state(tt_mma* mma, int k, const double* xo1, const double* xo2, const double* U, const double* L);
+
+/* Restart vectors of the physics (LinearElasticity::WriteRestartFiles, LinearElasticity.cc:2457-2496,
+   and their reading in SetUpSolver1, :2711-2723): the initial acceleration a0 (DDU0), the terminal
+   multiplier lambda_T (LT) and the t = 0 adjoint multiplier nu0 (DDL0), each nd values in host
+   memory.  get returns the latest ones (zeros before the first solve); set makes them the starting
+   guesses of the next tt_complete_initial_state (a0) and tt_objective_gradient (lambda_T, nu0), as the
+   reference's persistent vectors are.  A null pointer leaves that vector as it is. */
+int tt_get_restart_vectors(tt_ctx* ctx, double* a0, double* lamT, double* nu0);
+int tt_set_restart_vectors(tt_ctx* ctx, const double* a0, const double* lamT, const double* nu0);
+
+/* PETSc binary vector files, as PetscViewerBinary VecView / VecLoad write and read them for the
+   reference's checkpoint and restart files (LinearElasticity.cc:1128-1137, 1261-1276, 2474-2486;
+   TopOpt.cc:614-735) with 32-bit PetscInt: per vector the big-endian int32 class id 1211214, the
+   big-endian int32 length, then the values as big-endian IEEE doubles; vectors follow each other.
+   Natural ordering, which is this library's ordering.  Host memory only; no GPU work.  write
+   creates or truncates the file; read reads the first nvec vectors and fails (TT_ERR_ARG, message
+   from tt_last_error(NULL)) on a missing file, a wrong class id, a length other than lens[k] or a
+   short file. */
+int tt_petsc_write_vecs(const char* path, int nvec, const double* const* vecs, const long* lens);
+int tt_petsc_read_vecs(const char* path, int nvec, double* const* vecs, const long* lens);
 
 /* Measurement helper with no counterpart in the reference: nrep back-to-back launches of the
    tt_op_apply kernel on the context's stream, timed with HIP events on that stream; returns the

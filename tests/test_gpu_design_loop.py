@@ -38,7 +38,8 @@ def test_drop_in_design_loop_matches_oracle(tmp_path, precision, tol_f, tol_x):
     hist_file = tmp_path / "hist.jsonl"
     cmd = [DRIVER, *ARGS, "-precision", str(precision), "-history", str(hist_file)]
     print("running", " ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    # (run in the temporary directory: the driver writes the reference's restart files into ./)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=tmp_path)
     print(r.stdout[-3000:], r.stderr[-3000:])
     assert r.returncode == 0
     lines = [json.loads(s) for s in hist_file.read_text().splitlines() if s.strip()]

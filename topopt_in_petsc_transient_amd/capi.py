@@ -26,6 +26,8 @@ EXPORTED = [
     "tt_mma_set_asymptotes", "tt_mma_set_outer_movelimit", "tt_mma_update", "tt_mma_design_change",
     "tt_mma_kkt_residual", "tt_mma_get_asymptotes", "tt_get_stats", "tt_reset_stats", "tt_synchronize",
     "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
+    "tt_mma_get_state", "tt_mma_set_state", "tt_get_restart_vectors", "tt_set_restart_vectors",
+    "tt_petsc_write_vecs", "tt_petsc_read_vecs",
 ]
 
 
@@ -95,6 +97,11 @@ def lib():
             "tt_mma_get_asymptotes": [vp, vp, vp, dp],
             "tt_time_op_apply": [vp, ip, ip, vp, vp, ip, dp],
             "tt_get_stats": [vp, ctypes.POINTER(Stats)], "tt_reset_stats": [vp], "tt_synchronize": [vp],
+            "tt_mma_get_state": [vp, ctypes.POINTER(ctypes.c_int), vp, vp, vp, vp],
+            "tt_mma_set_state": [vp, ctypes.c_int, vp, vp, vp, vp],
+            "tt_get_restart_vectors": [vp, vp, vp, vp], "tt_set_restart_vectors": [vp, vp, vp, vp],
+            "tt_petsc_write_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
+            "tt_petsc_read_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -247,6 +254,40 @@ class Context:
     def synchronize(self):
         self._chk(lib().tt_synchronize(self.h))
 
+    def restart_vectors(self):
+        """(a0, lambda_T, nu0): the physics restart vectors (LinearElasticity.cc:2474-2486)."""
+        out = [np.zeros(self.nd) for _ in range(3)]
+        self._chk(lib().tt_get_restart_vectors(self.h, *[o.ctypes.data for o in out]))
+        return tuple(out)
+
+    def set_restart_vectors(self, a0=None, lamT=None, nu0=None):
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (a0, lamT, nu0)]
+        for v in arrs:
+            if v is not None and v.size != self.nd:
+                raise ValueError("restart vectors have nd values")
+        self._chk(lib().tt_set_restart_vectors(self.h, *[None if v is None else v.ctypes.data for v in arrs]))
+
+
+def petsc_write_vecs(path, vecs):
+    """Write vectors as a PETSc binary Vec file (VecView with a binary viewer, 32-bit PetscInt)."""
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in vecs]
+    ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    lens = (ctypes.c_long * max(1, len(arrs)))(*[a.size for a in arrs])
+    rc = lib().tt_petsc_write_vecs(os.fsencode(path), len(arrs), ptrs, lens)
+    if rc != 0:
+        raise TTError(rc, lib().tt_last_error(None).decode())
+
+
+def petsc_read_vecs(path, lens):
+    """Read the first len(lens) vectors of a PETSc binary Vec file (VecLoad), checking their lengths."""
+    out = [np.zeros(int(n)) for n in lens]
+    ptrs = (ctypes.c_void_p * max(1, len(out)))(*[a.ctypes.data for a in out])
+    ls = (ctypes.c_long * max(1, len(out)))(*[a.size for a in out])
+    rc = lib().tt_petsc_read_vecs(os.fsencode(path), len(out), ptrs, ls)
+    if rc != 0:
+        raise TTError(rc, lib().tt_last_error(None).decode())
+    return out
+
 
 class MMA:
     """MMA with one constraint on the context's GPU (MMA.cc)."""
@@ -298,6 +339,17 @@ class MMA:
                                                 arrs[2].ctypes.data, arrs[3].ctypes.data, arrs[4].ctypes.data,
                                                 ctypes.byref(n2), ctypes.byref(ni), 0))
         return n2.value, ni.value
+
+    def state(self):
+        """(k, xold1, xold2, U, L): the restart state of MMA::Restart (MMA.cc:307-350)."""
+        arrs = [np.zeros(self.n) for _ in range(4)]
+        k = ctypes.c_int()
+        self.ctx._chk(lib().tt_mma_get_state(self.h, ctypes.byref(k), *[a.ctypes.data for a in arrs]))
+        return (k.value, *arrs)
+
+    def set_state(self, k, xo1, xo2, U, L):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xo1, xo2, U, L)]
+        self.ctx._chk(lib().tt_mma_set_state(self.h, int(k), *[a.ctypes.data for a in arrs]))
 
     def asymptotes(self):
         L = np.zeros(self.n)

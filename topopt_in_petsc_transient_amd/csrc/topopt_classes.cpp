@@ -191,12 +191,13 @@ void TopOpt::Init(PetscInt nconstraint) {
     da_nodes = da_elem = nullptr;
     fx = 0.0;
     fscale = 1.0;
-    restart = PETSC_FALSE;
+    restart = PETSC_TRUE;  // TopOpt.cc:534: restart files are written by default
     flip = PETSC_TRUE;
+    xo1 = xo2 = U = L = nullptr;
     if (SetUp() != 0) std::exit(1);
 }
 TopOpt::~TopOpt() {
-    for (Vec* v : {&x, &xTilde, &xPhys, &dfdx, &xmin, &xmax, &xold})
+    for (Vec* v : {&x, &xTilde, &xPhys, &dfdx, &xmin, &xmax, &xold, &xo1, &xo2, &U, &L})
         if (*v) VecDestroy(v);
     if (dgdx) {
         for (PetscInt i = 0; i < m; i++) VecDestroy(&dgdx[i]);
@@ -348,20 +349,116 @@ PetscErrorCode TopOpt::SetUp() {
     return 0;
 }
 
-// MMA allocation with a = 0, c = 1000, d = 0 (TopOpt.cc:527-531); restart files are not read
-PetscErrorCode TopOpt::AllocateMMAwithRestart(PetscInt* itr, MMA** mma) {
-    PetscBool flg;
-    PetscOptionsGetBool(NULL, NULL, "-restart", &restart, &flg);
-    if (restart && flg) {
-        PetscPrintf(PETSC_COMM_WORLD, "# restart files are not supported by this build: starting from V0\n");
-        restart = PETSC_FALSE;
+static bool file_exists(const std::string& f) {
+    if (f.empty()) return false;
+    FILE* p = std::fopen(f.c_str(), "rb");
+    if (p) std::fclose(p);
+    return p != nullptr;
+}
+static PetscErrorCode write_vecs(const std::string& f, std::initializer_list<Vec> vs) {
+    std::vector<const double*> p;
+    std::vector<long> n;
+    for (Vec v : vs) {
+        p.push_back(v->a.data());
+        n.push_back((long)v->a.size());
     }
+    return report(nullptr, tt_petsc_write_vecs(f.c_str(), (int)p.size(), p.data(), n.data()), "tt_petsc_write_vecs");
+}
+static PetscErrorCode read_vecs(const std::string& f, std::initializer_list<Vec> vs) {
+    std::vector<double*> p;
+    std::vector<long> n;
+    for (Vec v : vs) {
+        p.push_back(v->a.data());
+        n.push_back((long)v->a.size());
+    }
+    return report(nullptr, tt_petsc_read_vecs(f.c_str(), (int)p.size(), p.data(), n.data()), "tt_petsc_read_vecs");
+}
+static std::string option_string(const char* name, const std::string& def) {
+    char buf[PETSC_MAX_PATH_LEN];
+    PetscBool flg = PETSC_FALSE;
+    PetscOptionsGetString(NULL, NULL, name, buf, sizeof(buf), &flg);
+    return flg ? std::string(buf) : def;
+}
+
+// MMA with a = 0, c = 1000, d = 0 (TopOpt.cc:527-531) and the restart point (TopOpt.cc:533-695):
+// -restart (default on: restart files are written), -onlyLoadDesign, -loadDesign (0: x, 2: xPhys),
+// -restartFileVec / -restartFileItr (the files read), -workdir (where Restart0x.dat and
+// Restart0x_itr_f0.dat are written)
+PetscErrorCode TopOpt::AllocateMMAwithRestart(PetscInt* itr, MMA** mma) {
     std::vector<PetscScalar> a(m, 0.0), c(m, 1000.0), d(m, 0.0);
+    restart = PETSC_TRUE;
+    flip = PETSC_TRUE;
+    PetscBool onlyLoadDesign = PETSC_FALSE, flg;
+    PetscInt loadDesign = 2;
+    PetscOptionsGetBool(NULL, NULL, "-restart", &restart, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-onlyLoadDesign", &onlyLoadDesign, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-loadDesign", &loadDesign, &flg);
+    if (restart) {
+        for (Vec* v : {&xo1, &xo2, &U, &L})
+            if (!*v) VecDuplicate(x, v);
+    }
+    const std::string wd = option_string("-workdir", "./");
+    filename00 = wd + "/Restart00.dat";
+    filename00Itr = wd + "/Restart00_itr_f0.dat";
+    filename01 = wd + "/Restart01.dat";
+    filename01Itr = wd + "/Restart01_itr_f0.dat";
+    const std::string fileVec = option_string("-restartFileVec", ""), fileItr = option_string("-restartFileItr", "");
+    PetscPrintf(PETSC_COMM_WORLD, "# Continue from previous iteration (-restart): %i \n", (int)restart);
+    PetscPrintf(PETSC_COMM_WORLD, "# Restart file (-restartFileVec): %s \n", fileVec.c_str());
+    PetscPrintf(PETSC_COMM_WORLD, "# Restart file (-restartFileItr): %s \n", fileItr.c_str());
+    const bool vecFile = file_exists(fileVec), itrFile = file_exists(fileItr);
+    if (!vecFile) PetscPrintf(PETSC_COMM_WORLD, "File: %s NOT FOUND \n", fileVec.c_str());
+    if (!itrFile) PetscPrintf(PETSC_COMM_WORLD, "File: %s NOT FOUND \n", fileItr.c_str());
     *itr = 0;
-    *mma = new MMA(n, m, x, a.data(), c.data(), d.data());
+    if (restart && vecFile && itrFile) {
+        // x, xPhys, xold1, xold2, U, L in the order TopOpt::WriteRestartFiles writes them
+        if (read_vecs(fileVec, {x, xPhys, xo1, xo2, U, L}) != 0) return TT_ERR_ARG;
+        if (onlyLoadDesign) {
+            if (loadDesign == 2) VecCopy(xPhys, x);  // the physical design as the initial design
+            PetscPrintf(PETSC_COMM_WORLD, "# Loading design from file: %s \n", fileVec.c_str());
+            *mma = new MMA(n, m, x, a.data(), c.data(), d.data());
+        } else {
+            VecCopy(x, xold);
+            FILE* f = std::fopen(fileItr.c_str(), "r");
+            int it = 0;
+            double fs = 0.0;
+            const bool ok = f && std::fscanf(f, "%d %lf", &it, &fs) == 2;
+            if (f) std::fclose(f);
+            if (!ok) {
+                std::fprintf(stderr, "[topopt] cannot read the iteration and fscale from %s\n", fileItr.c_str());
+                return TT_ERR_ARG;
+            }
+            *itr = it;
+            fscale = fs;
+            PetscPrintf(PETSC_COMM_WORLD, "# Continue optimization from file: %s \n", fileVec.c_str());
+            *mma = new MMA(n, m, *itr, xo1, xo2, U, L, a.data(), c.data(), d.data());
+        }
+        PetscPrintf(PETSC_COMM_WORLD, "# Successful restart from file: %s and %s \n", fileVec.c_str(),
+                    fileItr.c_str());
+    } else {
+        *mma = new MMA(n, m, x, a.data(), c.data(), d.data());
+    }
     return 0;
 }
-PetscErrorCode TopOpt::WriteRestartFiles(PetscInt*, MMA*) { return 0; }
+
+// TopOpt.cc:697-755: the iteration and fscale (ASCII) and x, xPhys, xold1, xold2, U, L (PETSc binary),
+// alternating between the 00 and 01 files
+PetscErrorCode TopOpt::WriteRestartFiles(PetscInt* itr, MMA* mma) {
+    if (!restart) return -1;
+    PetscErrorCode ierr = mma->Restart(xo1, xo2, U, L);
+    if (ierr) return ierr;
+    flip = flip ? PETSC_FALSE : PETSC_TRUE;
+    const std::string& fv = flip ? filename01 : filename00;
+    const std::string& fi = flip ? filename01Itr : filename00Itr;
+    FILE* f = std::fopen(fi.c_str(), "w");
+    if (!f || std::fprintf(f, "%d %1.16e\n", (int)itr[0], fscale) < 0) {
+        if (f) std::fclose(f);
+        std::fprintf(stderr, "[topopt] cannot write %s\n", fi.c_str());
+        return TT_ERR_ARG;
+    }
+    std::fclose(f);
+    return write_vecs(fv, {x, xPhys, xo1, xo2, U, L});
+}
 
 // ------------------------------------------------------------------ LinearElasticity
 LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt nlvls_in, PetscInt ncp_in,
@@ -414,6 +511,39 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     if (da_nodes->mesh->ctx) tt_destroy(da_nodes->mesh->ctx);  // a filter-only context is superseded
     da_nodes->mesh->ctx = ctx;
     g_current = ctx;
+    // restart point of the solves (SetUpSolver1, LinearElasticity.cc:2679-2723): with -restart (default
+    // on) and without -onlyLoadDesign, DDU0, LT and DDL0 are read from -restartFileVecSol and become the
+    // starting guesses of the initial-acceleration, terminal-adjoint and t = 0 adjoint solves
+    restart = PETSC_TRUE;
+    flip = PETSC_TRUE;
+    PetscBool onlyDesign = PETSC_FALSE, solutionAsU0 = PETSC_FALSE;
+    PetscOptionsGetBool(NULL, NULL, "-restart", &restart, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-onlyLoadDesign", &onlyDesign, &flg);
+    PetscOptionsGetBool(NULL, NULL, "-solutionAsU0", &solutionAsU0, &flg);
+    if (onlyDesign && solutionAsU0) {
+        std::fprintf(stderr, "[topopt] -solutionAsU0 (a nonzero initial displacement) is not part of this build\n");
+        std::exit(1);
+    }
+    const std::string wd = option_string("-workdir", "./");
+    filename00 = wd + "/RestartSol00.dat";
+    filename01 = wd + "/RestartSol01.dat";
+    if (restart && !onlyDesign) {
+        const std::string fileVec = option_string("-restartFileVecSol", "");
+        PetscPrintf(PETSC_COMM_WORLD, "# Restarting with solution (State Vector) from (-restartFileVecSol): %s \n",
+                    fileVec.c_str());
+        if (!file_exists(fileVec)) {
+            PetscPrintf(PETSC_COMM_WORLD, "File: %s NOT FOUND \n", fileVec.c_str());
+        } else {
+            const long nd = 3 * tt_num_nodes(ctx);
+            std::vector<double> a0(nd), lt(nd), dl0(nd);
+            double* v[3] = {a0.data(), lt.data(), dl0.data()};
+            const long lens[3] = {nd, nd, nd};
+            if (report(nullptr, tt_petsc_read_vecs(fileVec.c_str(), 3, v, lens), "tt_petsc_read_vecs") != TT_OK ||
+                report(ctx, tt_set_restart_vectors(ctx, a0.data(), lt.data(), dl0.data()), "tt_set_restart_vectors") !=
+                    TT_OK)
+                std::exit(1);
+        }
+    }
 }
 LinearElasticity::~LinearElasticity() {
     for (auto& s : state)
@@ -475,7 +605,20 @@ PetscErrorCode LinearElasticity::VolumeConstraint(PetscScalar* gx, Vec dgdx, Vec
     TTCALL(ctx, tt_volume_constraint(ctx, xPhys->a.data(), volfrac, gx, dgdx->a.data(), 0));
     return 0;
 }
-PetscErrorCode LinearElasticity::WriteRestartFiles() { return 0; }
+// LinearElasticity.cc:2457-2496: DDU0, LT, DDL0 (PETSc binary), alternating between the 00 and 01 files
+PetscErrorCode LinearElasticity::WriteRestartFiles() {
+    if (!restart) return -1;
+    flip = flip ? PETSC_FALSE : PETSC_TRUE;
+    const long nd = 3 * tt_num_nodes(ctx);
+    std::vector<double> a0(nd), lt(nd), dl0(nd);
+    TTCALL(ctx, tt_get_restart_vectors(ctx, a0.data(), lt.data(), dl0.data()));
+    const double* v[3] = {a0.data(), lt.data(), dl0.data()};
+    const long lens[3] = {nd, nd, nd};
+    const std::string& f = flip ? filename01 : filename00;
+    if (report(nullptr, tt_petsc_write_vecs(f.c_str(), 3, v, lens), "tt_petsc_write_vecs") != TT_OK) return TT_ERR_ARG;
+    PetscPrintf(PETSC_COMM_WORLD, flip ? "# Restart file written to 01-files.\n" : "# Restart file written to 00-files.\n");
+    return 0;
+}
 
 Vec LinearElasticity::StateField(PetscInt j, int which) {
     const long nd = 3 * tt_num_nodes(ctx);
@@ -575,6 +718,23 @@ MMA::MMA(PetscInt nn, PetscInt mm, Vec) : n(nn), m(mm), h(nullptr) { Create(0.0,
 MMA::MMA(PetscInt nn, PetscInt mm, Vec, PetscScalar* a, PetscScalar* c, PetscScalar* d) : n(nn), m(mm), h(nullptr) {
     Create(a[0], c[0], d[0]);
 }
+// restart constructors (MMA.cc:6-177): the iteration counter, the two previous iterates, the asymptotes
+MMA::MMA(PetscInt nn, PetscInt mm, PetscInt kk, Vec xo1, Vec xo2, Vec U, Vec L, PetscScalar* a, PetscScalar* c,
+         PetscScalar* d)
+    : n(nn), m(mm), h(nullptr) {
+    Create(a[0], c[0], d[0]);
+    if (kk < 3) PetscPrintf(PETSC_COMM_WORLD, "NOT A LEGAL RESTART POINT (k<3): EXPECT BREAKDOWN\n");
+    if (report(g_current, tt_mma_set_state(h, (int)kk, xo1->a.data(), xo2->a.data(), U->a.data(), L->a.data()),
+               "tt_mma_set_state") != TT_OK)
+        std::exit(1);
+}
+MMA::MMA(PetscInt nn, PetscInt mm, PetscInt kk, Vec xo1, Vec xo2, Vec U, Vec L) : n(nn), m(mm), h(nullptr) {
+    Create(0.0, 1000.0, 0.0);
+    if (kk < 3) PetscPrintf(PETSC_COMM_WORLD, "NOT A LEGAL RESTART POINT (k<3): EXPECT BREAKDOWN\n");
+    if (report(g_current, tt_mma_set_state(h, (int)kk, xo1->a.data(), xo2->a.data(), U->a.data(), L->a.data()),
+               "tt_mma_set_state") != TT_OK)
+        std::exit(1);
+}
 MMA::~MMA() {
     if (h) tt_mma_destroy(h);
 }
@@ -583,10 +743,11 @@ PetscErrorCode MMA::Update(Vec xval, Vec dfdx, PetscScalar* gx, Vec* dgdx, Vec x
                                     xmax->a.data(), 0));
     return 0;
 }
-PetscErrorCode MMA::Restart(Vec, Vec, Vec U, Vec L) {
-    double lam = 0.0;
-    TTCALL(g_current, tt_mma_get_asymptotes(h, L->a.data(), U->a.data(), &lam));
-    return TT_ERR_STATE;  // previous iterates are not exported: no restart files in this build
+// MMA.cc:307-350: copies of the previous iterates and the asymptotes
+PetscErrorCode MMA::Restart(Vec xo1, Vec xo2, Vec U, Vec L) {
+    int k = 0;
+    TTCALL(g_current, tt_mma_get_state(h, &k, xo1->a.data(), xo2->a.data(), U->a.data(), L->a.data()));
+    return 0;
 }
 PetscErrorCode MMA::SetAsymptotes(PetscScalar init, PetscScalar decrease, PetscScalar increase) {
     TTCALL(g_current, tt_mma_set_asymptotes(h, init, decrease, increase));

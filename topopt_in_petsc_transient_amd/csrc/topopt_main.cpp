@@ -82,7 +82,15 @@ int main(int argc, char* argv[]) {
             }
             if (opt->gx[0] > 0) ch = 1;
             CHK(filter->FilterProject(opt->x, opt->xTilde, opt->xPhys, opt->projectionFilter, opt->beta, opt->eta));
+            // restart files every 10 iterations and when beta changes (main.cc:156-160)
+            if (itr % 10 == 0 || change_beta) {
+                opt->WriteRestartFiles(&itr, mma);
+                physics->WriteRestartFiles();
+            }
         }
+        // and at the end (main.cc:163-165)
+        opt->WriteRestartFiles(&itr, mma);
+        physics->WriteRestartFiles();
         if (hist) {
             // final design (xPhys) for comparison
             std::fprintf(hist, "{\"xphys\": [");

@@ -1,6 +1,8 @@
 // Context set-up, PDE filter, MMA driver and the extern "C" entry points.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 
@@ -172,6 +174,7 @@ Context::Context(const tt_params& p) : prm(p) {
              "state storage exceeds device memory: lower checkpoint_slots");
     slots.resize(nslots);
     for (auto& s : slots) s.alloc((size_t)kStateVecs * nd);
+    TT_HIP(hipMemsetAsync(slots[0].p, 0, (size_t)kStateVecs * nd * 8, stream));  // a0 reads as 0 before the first solve
     slot_step.assign(nslots, -1);
     st_cur.alloc((size_t)kStateVecs * nd);
     st_next.alloc((size_t)kStateVecs * nd);
@@ -744,6 +747,121 @@ int tt_mma_get_asymptotes(tt_mma* m, double* L, double* U, double* lam) {
     C.sync();
     if (lam) *lam = m->lam;
     TT_API_END
+}
+
+int tt_mma_get_state(tt_mma* m, int* k, double* xo1, double* xo2, double* U, double* L) {
+    TT_MMA_BEGIN(m)
+    const size_t bytes = m->n * sizeof(double);
+    if (xo1) TT_HIP(hipMemcpyAsync(xo1, m->xo1.p, bytes, hipMemcpyDeviceToHost, C.stream));
+    if (xo2) TT_HIP(hipMemcpyAsync(xo2, m->xo2.p, bytes, hipMemcpyDeviceToHost, C.stream));
+    if (U) TT_HIP(hipMemcpyAsync(U, m->U.p, bytes, hipMemcpyDeviceToHost, C.stream));
+    if (L) TT_HIP(hipMemcpyAsync(L, m->L.p, bytes, hipMemcpyDeviceToHost, C.stream));
+    C.sync();
+    if (k) *k = m->k;
+    TT_API_END
+}
+
+int tt_mma_set_state(tt_mma* m, int k, const double* xo1, const double* xo2, const double* U, const double* L) {
+    TT_MMA_BEGIN(m)
+    TT_CHECK(xo1 && xo2 && U && L && k >= 0, TT_ERR_ARG, "bad MMA restart state");
+    const size_t bytes = m->n * sizeof(double);
+    TT_HIP(hipMemcpyAsync(m->xo1.p, xo1, bytes, hipMemcpyHostToDevice, C.stream));
+    TT_HIP(hipMemcpyAsync(m->xo2.p, xo2, bytes, hipMemcpyHostToDevice, C.stream));
+    TT_HIP(hipMemcpyAsync(m->U.p, U, bytes, hipMemcpyHostToDevice, C.stream));
+    TT_HIP(hipMemcpyAsync(m->L.p, L, bytes, hipMemcpyHostToDevice, C.stream));
+    C.sync();
+    m->k = k;
+    TT_API_END
+}
+
+int tt_get_restart_vectors(tt_ctx* ctx, double* a0, double* lamT, double* nu0) {
+    TT_API_BEGIN(ctx)
+    const size_t bytes = C.g.nd * sizeof(double);
+    if (a0) TT_HIP(hipMemcpyAsync(a0, C.slots[0].p + 2 * C.g.nd, bytes, hipMemcpyDeviceToHost, C.stream));
+    if (lamT) TT_HIP(hipMemcpyAsync(lamT, C.lamT.p, bytes, hipMemcpyDeviceToHost, C.stream));
+    if (nu0) TT_HIP(hipMemcpyAsync(nu0, C.nu0.p, bytes, hipMemcpyDeviceToHost, C.stream));
+    C.sync();
+    TT_API_END
+}
+
+int tt_set_restart_vectors(tt_ctx* ctx, const double* a0, const double* lamT, const double* nu0) {
+    TT_API_BEGIN(ctx)
+    const size_t bytes = C.g.nd * sizeof(double);
+    if (a0) {
+        TT_HIP(hipMemcpyAsync(C.slots[0].p + 2 * C.g.nd, a0, bytes, hipMemcpyHostToDevice, C.stream));
+        C.a0_guess = true;
+    }
+    if (lamT) TT_HIP(hipMemcpyAsync(C.lamT.p, lamT, bytes, hipMemcpyHostToDevice, C.stream));
+    if (nu0) TT_HIP(hipMemcpyAsync(C.nu0.p, nu0, bytes, hipMemcpyHostToDevice, C.stream));
+    C.sync();
+    TT_API_END
+}
+
+// ---- PETSc binary vectors (VecView / VecLoad with a binary viewer, 32-bit PetscInt): big-endian
+// int32 class id, big-endian int32 length, big-endian doubles
+static constexpr int32_t kPetscVecClassId = 1211214;
+
+static int petsc_io_error(const std::string& msg) {
+    g_create_error = msg;
+    return TT_ERR_ARG;
+}
+
+int tt_petsc_write_vecs(const char* path, int nvec, const double* const* vecs, const long* lens) {
+    if (!path || nvec < 0 || (nvec > 0 && (!vecs || !lens))) return petsc_io_error("tt_petsc_write_vecs: bad arguments");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return petsc_io_error(std::string("tt_petsc_write_vecs: cannot open ") + path);
+    std::vector<uint64_t> buf;
+    bool ok = true;
+    for (int k = 0; k < nvec && ok; k++) {
+        if (lens[k] < 0 || lens[k] > INT32_MAX || (lens[k] > 0 && !vecs[k])) {
+            ok = false;
+            break;
+        }
+        const uint32_t head[2] = {__builtin_bswap32((uint32_t)kPetscVecClassId), __builtin_bswap32((uint32_t)lens[k])};
+        buf.resize(lens[k]);
+        for (long i = 0; i < lens[k]; i++) {
+            uint64_t b;
+            std::memcpy(&b, vecs[k] + i, 8);
+            buf[i] = __builtin_bswap64(b);
+        }
+        ok = std::fwrite(head, 4, 2, f) == 2 && std::fwrite(buf.data(), 8, lens[k], f) == (size_t)lens[k];
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return petsc_io_error(std::string("tt_petsc_write_vecs: write failed for ") + path);
+    return TT_OK;
+}
+
+int tt_petsc_read_vecs(const char* path, int nvec, double* const* vecs, const long* lens) {
+    if (!path || nvec < 0 || (nvec > 0 && (!vecs || !lens))) return petsc_io_error("tt_petsc_read_vecs: bad arguments");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return petsc_io_error(std::string("tt_petsc_read_vecs: cannot open ") + path);
+    std::vector<uint64_t> buf;
+    std::string err;
+    for (int k = 0; k < nvec && err.empty(); k++) {
+        uint32_t head[2];
+        if (std::fread(head, 4, 2, f) != 2) {
+            err = "file ends before vector " + std::to_string(k);
+        } else if ((int32_t)__builtin_bswap32(head[0]) != kPetscVecClassId) {
+            err = "vector " + std::to_string(k) + " has class id " + std::to_string((int32_t)__builtin_bswap32(head[0])) +
+                  ", not the Vec class id 1211214 (a file written with 64-bit PetscInt or not a Vec file?)";
+        } else if ((long)__builtin_bswap32(head[1]) != lens[k]) {
+            err = "vector " + std::to_string(k) + " has length " + std::to_string(__builtin_bswap32(head[1])) +
+                  ", expected " + std::to_string(lens[k]);
+        } else {
+            buf.resize(lens[k]);
+            if (std::fread(buf.data(), 8, lens[k], f) != (size_t)lens[k]) {
+                err = "file ends inside vector " + std::to_string(k);
+            } else {
+                for (long i = 0; i < lens[k]; i++) {
+                    const uint64_t b = __builtin_bswap64(buf[i]);
+                    std::memcpy(vecs[k] + i, &b, 8);
+                }
+            }
+        }
+    }
+    std::fclose(f);
+    if (!err.empty()) return petsc_io_error(std::string("tt_petsc_read_vecs: ") + path + ": " + err);
+    return TT_OK;
 }
 
 int tt_get_stats(tt_ctx* ctx, tt_stats* st) {

@@ -102,6 +102,7 @@ struct Context {
     DBuf<float> kf32, mf32;  // FP32 fine coefficients for the FP32 microbenchmark apply
     bool density_set = false;
     bool initial_done = false;
+    bool a0_guess = false;  // slot 0 holds a starting guess for a0 (tt_set_restart_vectors), used once
     bool forward_done = false;
 
     Multigrid mg;

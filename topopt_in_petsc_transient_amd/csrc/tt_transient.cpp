@@ -43,7 +43,12 @@ void Context::complete_initial_state() {
     launch_diag_inv(op_fine64, 0.0, 1.0, dinvM.p, true, stream);
     // state 0: u0 = v0 = 0 (LinearElasticity.cc:521-525); M~ a0 = N.(f(0) - C v0 - K u0) (:1050-1078)
     double* s0 = slots[0].p;
-    launch_fill(s0, 0.0, kStateVecs * nd, true, stream);  // u0 = v0 = 0, the solve's initial guess a0 = 0
+    // u0 = v0 = 0 and no acceleration history; the solve starts from a0 = 0, or once from the a0 that
+    // tt_set_restart_vectors supplied (the reference's DDU0 read from RestartSol0x.dat, :2716-2720)
+    launch_fill(s0, 0.0, 2 * nd, true, stream);
+    launch_fill(s0 + 3 * nd, 0.0, (kStateVecs - 3) * nd, true, stream);
+    if (!a0_guess) launch_fill(s0 + 2 * nd, 0.0, nd, true, stream);
+    a0_guess = false;
     const double wK[3] = {-1.0, -prm.dbeta, 0.0};
     const double wM[3] = {0.0, -prm.dalpha, 0.0};
     launch_newmark_rhs(op_fine64, s0, s0 + nd, s0 + 2 * nd, F.p, load_intensity(0), wK, wM, rhs.p, stream);
