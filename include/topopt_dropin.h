@@ -50,8 +50,11 @@ struct _p_DM {
 };
 typedef _p_DM* DM;
 
-/* MPIIO.h:35-46: the output writer.  Field output is not part of this build: the object can be
-   constructed (so main.cc compiles unchanged), prints one notice, and WriteVTK writes nothing. */
+/* MPIIO.h / MPIIO.cc: the single-rank form of the reference's field output (bin2vtu.py reads it).
+   One domain of hex8 cells; the file is the info line, the header (unsigned long: domains, points,
+   cells, point fields, cell fields, nodes per element), the field names, float32 point coordinates,
+   the connectivity, offsets and VTK types (unsigned long), then per WriteVTK call the step number
+   (unsigned long) and the float32 point and cell fields, one field after the other. */
 class MPIIO {
    public:
     MPIIO(DM da_nodes, int nPfields, std::string pnames, int nCfields, std::string cnames, std::string filename);
@@ -60,6 +63,15 @@ class MPIIO {
     PetscErrorCode WriteVTK(DM da_nodes, Vec U, Vec x, Vec xTilde, Vec xPhys, PetscInt itr);
     PetscErrorCode WriteVTK(DM da_nodes, Vec U, Vec xPhys, PetscInt itr);
     PetscErrorCode WriteVTK(DM da_nodes, Vec U, Vec DU, Vec DDU, Vec x, Vec xtilde, Vec xPhys, PetscInt itr);
+
+   private:
+    std::string filename;
+    unsigned long nPoints, nCells;
+    int nPF, nCF;
+    std::vector<float> workPoint, workCell;  // nPF x nPoints and nCF x nCells, one field after the other
+    PetscErrorCode Append(PetscInt itr);
+    void PutPoint(int field0, Vec v, int ncomp);
+    void PutCell(int field, Vec v);
 };
 
 PetscErrorCode PetscInitialize(int* argc, char*** argv, const char* file, const char* help);
@@ -163,9 +175,10 @@ class LinearElasticity {
     PetscInt Getts() { return Ts; }
 
    private:
-    PetscInt Ts;
+    PetscInt Ts, ncp;
     tt_ctx* ctx;
     bool forward_done;
+    PetscErrorCode WriteStates(MPIIO* output, Vec xPhys);
     // restart (LinearElasticity.cc:2457-2496, 2679-2723): two alternating RestartSol files
     PetscBool restart, flip;
     std::string filename00, filename01;

@@ -123,7 +123,9 @@ int tt_complete_initial_state(tt_ctx* ctx);
    whole Newmark forward sweep, keeping binomial checkpoints (or every state). */
 int tt_forward(tt_ctx* ctx);
 
-/* State at step t (0..nsteps) when every state is stored; 3*nn doubles each (may be NULL). */
+/* State at step t (0..nsteps) of the last tt_forward (GetStateFieldU/DU/DDU, LinearElasticity.h:68-78);
+   3*nn doubles each (may be NULL).  A state that the binomial checkpoints did not keep is recomputed
+   from the nearest stored one; asking for t = 1, 2, ... in turn costs one Newmark step each. */
 int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device);
 
 /* ComputeObjectiveSensitivities (LinearElasticity.cc:1625-1908): objective fx and dfdx

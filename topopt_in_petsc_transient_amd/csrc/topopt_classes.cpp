@@ -8,6 +8,7 @@
 
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -121,15 +122,124 @@ PetscErrorCode DMDestroy(DM* dm) {
 }
 
 // ------------------------------------------------------------------ output writer (not part of this build)
-MPIIO::MPIIO(DM, int, std::string, int, std::string, std::string filename) {
-    std::fprintf(stderr, "[topopt] field output (%s) is not part of this build: nothing is written\n",
-                 filename.c_str());
+// MPIIO.cc:32-136, 409-586 (constructor, Allocate, writePoints, writeCells) for one rank and one domain:
+// all of it is the file's first part, written here in one go.  With -workdir the file goes to
+// "./" + workdir + "/" + filename, as MPIIO.cc:431-437 builds it; an absolute workdir (which that
+// prefix would turn into a relative path) is used as it is.
+MPIIO::MPIIO(DM da, int nPfields, std::string pnames, int nCfields, std::string cnames, std::string fname) {
+    char wd[PETSC_MAX_PATH_LEN];
+    PetscBool flg = PETSC_FALSE;
+    PetscOptionsGetString(NULL, NULL, "-workdir", wd, sizeof(wd), &flg);
+    filename = !flg ? fname : (wd[0] == '/' ? std::string(wd) : std::string("./") + wd) + "/" + fname;
+    PetscPrintf(PETSC_COMM_WORLD, "# Outputfile is written to: %s \n", filename.c_str());
+    const long nx = da->nxyz[0], ny = da->nxyz[1], nz = da->nxyz[2];
+    nPoints = (unsigned long)(nx * ny * nz);
+    nCells = (unsigned long)((nx - 1) * (ny - 1) * (nz - 1));
+    nPF = nPfields;
+    nCF = nCfields;
+    workPoint.assign((size_t)std::max(nPF, 0) * nPoints, 0.0f);
+    workCell.assign((size_t)std::max(nCF, 0) * nCells, 0.0f);
+    std::string head = std::string("TopOpt result version 1.1") + "\n\x01";
+    const unsigned long header[6] = {1, nPoints, nCells, (unsigned long)nPF, (unsigned long)nCF, 8};
+    const std::string names = pnames + "\x01" + cnames + "\x01";
+    // node coordinates (natural order, x fastest), as float32
+    std::vector<float> pts(3 * nPoints);
+    const double h[3] = {(da->xc[1] - da->xc[0]) / (nx - 1), (da->xc[3] - da->xc[2]) / (ny - 1),
+                         (da->xc[5] - da->xc[4]) / (nz - 1)};
+    for (long k = 0, n = 0; k < nz; k++)
+        for (long j = 0; j < ny; j++)
+            for (long i = 0; i < nx; i++, n++) {
+                pts[3 * n] = (float)(da->xc[0] + i * h[0]);
+                pts[3 * n + 1] = (float)(da->xc[2] + j * h[1]);
+                pts[3 * n + 2] = (float)(da->xc[4] + k * h[2]);
+            }
+    // hex8 cells in the element corner order (LinearElasticity.cc:3067-3074), offsets, VTK type 12
+    std::vector<unsigned long> conn(8 * nCells), offs(nCells), types(nCells, 12);
+    for (long k = 0, e = 0; k < nz - 1; k++)
+        for (long j = 0; j < ny - 1; j++)
+            for (long i = 0; i < nx - 1; i++, e++) {
+                const unsigned long n0 = i + nx * (j + ny * k), sy = nx, sz = nx * ny;
+                const unsigned long c[8] = {n0, n0 + 1, n0 + 1 + sy, n0 + sy, n0 + sz, n0 + 1 + sz, n0 + 1 + sy + sz,
+                                            n0 + sy + sz};
+                for (int q = 0; q < 8; q++) conn[8 * e + q] = c[q];
+                offs[e] = 8 * (e + 1);
+            }
+    FILE* f = std::fopen(filename.c_str(), "wb");
+    bool ok = f != nullptr;
+    ok = ok && std::fwrite(head.data(), 1, head.size(), f) == head.size();
+    ok = ok && std::fwrite(header, sizeof(unsigned long), 6, f) == 6;
+    ok = ok && std::fwrite(names.data(), 1, names.size(), f) == names.size();
+    ok = ok && std::fwrite(pts.data(), sizeof(float), pts.size(), f) == pts.size();
+    ok = ok && std::fwrite(conn.data(), sizeof(unsigned long), conn.size(), f) == conn.size();
+    ok = ok && std::fwrite(offs.data(), sizeof(unsigned long), offs.size(), f) == offs.size();
+    ok = ok && std::fwrite(types.data(), sizeof(unsigned long), types.size(), f) == types.size();
+    if (f) ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {  // the reference aborts here (MPIIO.cc:536-538)
+        std::fprintf(stderr, "[topopt] cannot write the output file %s\n", filename.c_str());
+        std::exit(1);
+    }
 }
 MPIIO::~MPIIO() {}
-PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, Vec, PetscInt) { return 0; }
-PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, Vec, Vec, PetscInt) { return 0; }
-PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, PetscInt) { return 0; }
-PetscErrorCode MPIIO::WriteVTK(DM, Vec, Vec, Vec, Vec, Vec, Vec, PetscInt) { return 0; }
+// one dataset (writePointFields + writeCellFields, MPIIO.cc:724-905): the step, nPF point fields,
+// nCF cell fields
+PetscErrorCode MPIIO::Append(PetscInt itr) {
+    const unsigned long step = (unsigned long)itr;
+    FILE* f = std::fopen(filename.c_str(), "ab");
+    bool ok = f != nullptr;
+    ok = ok && std::fwrite(&step, sizeof(step), 1, f) == 1;
+    ok = ok && std::fwrite(workPoint.data(), sizeof(float), workPoint.size(), f) == workPoint.size();
+    ok = ok && std::fwrite(workCell.data(), sizeof(float), workCell.size(), f) == workCell.size();
+    if (f) ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        std::fprintf(stderr, "[topopt] cannot append to the output file %s\n", filename.c_str());
+        return TT_ERR_ARG;
+    }
+    return 0;
+}
+// point fields field0 .. field0 + ncomp - 1 from the components of a nodal vector (fields past nPF
+// are not written, as MPIIO.cc writes only nPFields of its work array)
+void MPIIO::PutPoint(int field0, Vec v, int ncomp) {
+    for (int c = 0; c < ncomp; c++) {
+        const int fld = field0 + c;
+        if (fld >= nPF) return;
+        for (unsigned long i = 0; i < nPoints; i++) workPoint[fld * nPoints + i] = (float)v->a[ncomp * i + c];
+    }
+}
+void MPIIO::PutCell(int field, Vec v) {
+    if (field >= nCF) return;
+    for (unsigned long i = 0; i < nCells; i++) workCell[field * nCells + i] = (float)v->a[i];
+}
+// MPIIO.cc:373-404: the design fields (x, xTilde, xPhys); the point fields keep their last values
+PetscErrorCode MPIIO::WriteVTK(DM, Vec x, Vec xTilde, Vec xPhys, PetscInt itr) {
+    PutCell(0, x);
+    PutCell(1, xTilde);
+    PutCell(2, xPhys);
+    return Append(itr);
+}
+// MPIIO.cc:152-213: displacement (ux, uy, uz) and x, xTilde, xPhys
+PetscErrorCode MPIIO::WriteVTK(DM, Vec U, Vec x, Vec xTilde, Vec xPhys, PetscInt itr) {
+    PutPoint(0, U, 3);
+    PutCell(0, x);
+    PutCell(1, xTilde);
+    PutCell(2, xPhys);
+    return Append(itr);
+}
+// MPIIO.cc:215-268: displacement (ux, uy, uz) and xPhys
+PetscErrorCode MPIIO::WriteVTK(DM, Vec U, Vec xPhys, PetscInt itr) {
+    PutPoint(0, U, 3);
+    PutCell(0, xPhys);
+    return Append(itr);
+}
+// MPIIO.cc:270-371: displacement, velocity and acceleration, and x, xTilde, xPhys
+PetscErrorCode MPIIO::WriteVTK(DM, Vec U, Vec DU, Vec DDU, Vec x, Vec xTilde, Vec xPhys, PetscInt itr) {
+    PutPoint(0, U, 3);
+    PutPoint(3, DU, 3);
+    PutPoint(6, DDU, 3);
+    PutCell(0, x);
+    PutCell(1, xTilde);
+    PutCell(2, xPhys);
+    return Append(itr);
+}
 
 // ------------------------------------------------------------------ shared GPU context of a mesh
 struct TTMesh {
@@ -257,7 +367,7 @@ PetscErrorCode TopOpt::SetUp() {
     reduction = PETSC_FALSE;
     basis = 5;
     omega0 = 1.0;
-    outputPhysics0 = outputPhysics1 = outputDesign = PETSC_FALSE;
+    outputPhysics0 = outputPhysics1 = outputDesign = PETSC_TRUE;  // TopOpt.cc:474-476
     // mesh (SetUpMESH, TopOpt.cc:175-330)
     PetscOptionsGetInt(NULL, NULL, "-nx", &nxyz[0], &flg);
     PetscOptionsGetInt(NULL, NULL, "-ny", &nxyz[1], &flg);
@@ -472,6 +582,7 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     for (auto& s : state) s = nullptr;
     for (auto& v : mat) v = std::nan("");
     Ts = ncp_in * tsc_in + 1;  // LinearElasticity.cc:96
+    ncp = ncp_in;
     if (reduction_in) {
         std::fprintf(stderr, "[topopt] the reduced-order (-reduction) path is not part of this build\n");
         std::exit(1);
@@ -564,19 +675,42 @@ PetscErrorCode LinearElasticity::SetMaterial(PetscScalar Emin, PetscScalar Emax,
 // LinearElasticity.cc:956-1206
 PetscErrorCode LinearElasticity::CompleteInitialState(Vec xPhys, PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin,
                                                       PetscScalar Rmax, PetscScalar penalK, PetscScalar penalM,
-                                                      PetscScalar dalpha, PetscScalar dbeta, PetscBool, MPIIO*) {
+                                                      PetscScalar dalpha, PetscScalar dbeta, PetscBool write,
+                                                      MPIIO* output) {
     PetscErrorCode ierr = SetMaterial(Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta);
     if (ierr) return ierr;
     TTCALL(ctx, tt_set_density(ctx, xPhys->a.data(), 0));
     TTCALL(ctx, tt_complete_initial_state(ctx));
     forward_done = false;
+    if (write && output) {  // the initial state (:1086-1113): u0 = 0
+        Vec U0;
+        VecCreateSeq(0, 3 * (PetscInt)tt_num_nodes(ctx), &U0);
+        ierr = output->WriteVTK(da_nodal, U0, xPhys, 0);
+        VecDestroy(&U0);
+        if (ierr) return ierr;
+    }
     return 0;
 }
 
+// displacements of steps 1 .. Ts-1 to the field output (LinearElasticity.cc:1248-1250, 1737-1742);
+// states the checkpoints did not keep are recomputed in order, one Newmark step each
+PetscErrorCode LinearElasticity::WriteStates(MPIIO* output, Vec xPhys) {
+    Vec U;
+    VecCreateSeq(0, 3 * (PetscInt)tt_num_nodes(ctx), &U);
+    PetscErrorCode ierr = 0;
+    for (PetscInt t = 1; t < Ts && !ierr; t++) {
+        ierr = report(ctx, tt_get_state(ctx, t, U->a.data(), nullptr, nullptr, 0), "tt_get_state");
+        if (!ierr) ierr = output->WriteVTK(da_nodal, U, xPhys, t);
+    }
+    VecDestroy(&U);
+    return ierr;
+}
+
 // LinearElasticity.cc:1208-1280 (binomial checkpoints in device memory instead of files)
-PetscErrorCode LinearElasticity::GenerateCheckPoints(PetscBool, Vec, MPIIO*) {
+PetscErrorCode LinearElasticity::GenerateCheckPoints(PetscBool write, Vec xPhys, MPIIO* output) {
     TTCALL(ctx, tt_forward(ctx));
     forward_done = true;
+    if (write && output) return WriteStates(output, xPhys);
     return 0;
 }
 
@@ -585,7 +719,7 @@ PetscErrorCode LinearElasticity::ComputeObjectiveSensitivities(PetscScalar* fx, 
                                                                PetscScalar Emin, PetscScalar Emax, PetscScalar Rmin,
                                                                PetscScalar Rmax, PetscScalar penalK,
                                                                PetscScalar penalM, PetscScalar dalpha,
-                                                               PetscScalar dbeta, PetscBool, MPIIO*) {
+                                                               PetscScalar dbeta, PetscBool write, MPIIO* output) {
     const double m[8] = {Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta};
     if (std::memcmp(m, mat, sizeof(m)) != 0) {  // new material: redo the initial state as the reference would
         PetscErrorCode ierr = CompleteInitialState(xPhys, Emin, Emax, Rmin, Rmax, penalK, penalM, dalpha, dbeta,
@@ -595,6 +729,10 @@ PetscErrorCode LinearElasticity::ComputeObjectiveSensitivities(PetscScalar* fx, 
     if (!forward_done) {
         TTCALL(ctx, tt_forward(ctx));
         forward_done = true;
+    }
+    if (write && ncp == 1 && output) {  // :1736-1742
+        PetscErrorCode ierr = WriteStates(output, xPhys);
+        if (ierr) return ierr;
     }
     TTCALL(ctx, tt_objective_gradient(ctx, fx, dfdx->a.data(), 0));
     return 0;

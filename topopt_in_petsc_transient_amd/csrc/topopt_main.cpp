@@ -19,6 +19,15 @@ int main(int argc, char* argv[]) {
                              opt->objective, opt->sInt, opt->mInt, opt->t0, opt->boundaries, opt->loadloc,
                              opt->reduction, opt->basis, opt->dalpha, opt->dbeta, opt->omega0,
                              opt->optimizationRegion, opt->V0);
+    // field output (main.cc:39-49; -outputDesign, -outputPhysics0, -outputPhysics1, all on by default)
+    MPIIO* output1 = opt->outputDesign ? new MPIIO(opt->da_nodes, 0, "", 3, "x, xTilde, xPhys", "outputDesign_00000.dat")
+                                       : nullptr;
+    MPIIO* output2 = opt->outputPhysics0
+                         ? new MPIIO(opt->da_nodes, 3, "ux, uy, uz", 1, "xPhys", "outputPhysics0_00000.dat")
+                         : nullptr;
+    MPIIO* output3 = opt->outputPhysics1
+                         ? new MPIIO(opt->da_nodes, 3, "ux, uy, uz", 1, "xPhys", "outputPhysics1_00000.dat")
+                         : nullptr;
     Filter* filter = new Filter(opt->da_nodes, opt->xPhys, opt->filter, opt->rmin);
     MMA* mma = nullptr;
     PetscInt itr = 0;
@@ -46,15 +55,16 @@ int main(int argc, char* argv[]) {
             itr++;
             const double t_start = MPI_Wtime();
             CHK(physics->CompleteInitialState(opt->xPhys, opt->Emin, opt->Emax, opt->Rmin, opt->Rmax, opt->penalK,
-                                              opt->penalM, opt->dalpha, opt->dbeta, first, nullptr));
-            if (opt->ncp > 1) CHK(physics->GenerateCheckPoints(first, opt->xPhys, nullptr));
+                                              opt->penalM, opt->dalpha, opt->dbeta, first, output2));
+            if (opt->ncp > 1) CHK(physics->GenerateCheckPoints(first, opt->xPhys, output2));
             CHK(physics->ComputeObjectiveSensitivities(&opt->fx, opt->dfdx, opt->xPhys, opt->Emin, opt->Emax,
                                                        opt->Rmin, opt->Rmax, opt->penalK, opt->penalM, opt->dalpha,
-                                                       opt->dbeta, first, nullptr));
+                                                       opt->dbeta, first, output2));
             CHK(physics->VolumeConstraint(&opt->gx[0], opt->dgdx[0], opt->xPhys, opt->volfrac));
             CHK(filter->Gradients(opt->x, opt->xTilde, opt->dfdx, opt->m, opt->dgdx, opt->projectionFilter,
                                   opt->beta, opt->eta));
             first = PETSC_FALSE;
+            if (output1) CHK(output1->WriteVTK(physics->da_nodal, opt->x, opt->xTilde, opt->xPhys, itr));
             const PetscScalar mnd = filter->GetMND(opt->xPhys);
             if (itr == 1) opt->fscale = 1.0 / opt->fx;
             opt->fx *= opt->fscale;
@@ -91,6 +101,13 @@ int main(int argc, char* argv[]) {
         // and at the end (main.cc:163-165)
         opt->WriteRestartFiles(&itr, mma);
         physics->WriteRestartFiles();
+        // the final design's physics once more, into outputPhysics1 (main.cc:167-176)
+        CHK(physics->CompleteInitialState(opt->xPhys, opt->Emin, opt->Emax, opt->Rmin, opt->Rmax, opt->penalK,
+                                          opt->penalM, opt->dalpha, opt->dbeta, PETSC_TRUE, output3));
+        if (opt->ncp > 1) CHK(physics->GenerateCheckPoints(PETSC_TRUE, opt->xPhys, output3));
+        CHK(physics->ComputeObjectiveSensitivities(&opt->fx, opt->dfdx, opt->xPhys, opt->Emin, opt->Emax, opt->Rmin,
+                                                   opt->Rmax, opt->penalK, opt->penalM, opt->dalpha, opt->dbeta,
+                                                   PETSC_TRUE, output3));
         if (hist) {
             // final design (xPhys) for comparison
             std::fprintf(hist, "{\"xphys\": [");
@@ -103,6 +120,9 @@ int main(int argc, char* argv[]) {
 done:
     if (hist) std::fclose(hist);
     delete mma;
+    delete output1;
+    delete output2;
+    delete output3;
     delete filter;
     delete physics;
     delete opt;

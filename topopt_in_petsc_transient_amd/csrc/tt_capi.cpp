@@ -554,16 +554,12 @@ int tt_forward(tt_ctx* ctx) {
 
 int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device) {
     TT_API_BEGIN(ctx)
-    TT_CHECK(t >= 0 && t <= C.prm.nsteps, TT_ERR_ARG, "time step out of range");
-    int k = -1;
-    for (size_t i = 0; i < C.slot_step.size(); i++)
-        if (C.slot_step[i] == t) k = (int)i;
-    TT_CHECK(k >= 0, TT_ERR_STATE, "state not stored (use checkpoint_slots >= nsteps to keep every state)");
+    const double* st = C.state_at(t);
     const long nd = C.g.nd;
     double* outs[3] = {u, v, a};
     for (int q = 0; q < 3; q++) {
         if (!outs[q]) continue;
-        TT_HIP(hipMemcpyAsync(outs[q], C.slots[k].p + q * nd, nd * 8,
+        TT_HIP(hipMemcpyAsync(outs[q], st + q * nd, nd * 8,
                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, C.stream));
     }
     C.sync();

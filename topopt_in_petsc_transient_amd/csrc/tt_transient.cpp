@@ -201,8 +201,40 @@ struct Revolve {
     }
 };
 
+// State t of the last forward sweep: a stored one (checkpoint slot, or state T), else recomputed in
+// st_cur from the nearest stored state below t, or from the state recomputed last when that is
+// nearer, so that asking for t = 1, 2, ..., T in turn costs one Newmark step each.
+const double* Context::state_at(int t) {
+    TT_CHECK(forward_done, TT_ERR_STATE, "tt_forward must be called first");
+    TT_CHECK(t >= 0 && t <= prm.nsteps, TT_ERR_ARG, "time step out of range");
+    int base = -1, kb = -1;
+    for (size_t i = 0; i < slot_step.size(); i++) {
+        if (slot_step[i] == t) return slots[i].p;
+        if (slot_step[i] >= 0 && slot_step[i] < t && slot_step[i] > base) {
+            base = slot_step[i];
+            kb = (int)i;
+        }
+    }
+    if (!store_all && t == prm.nsteps) return st_last.p;
+    TT_CHECK(kb >= 0, TT_ERR_INTERNAL, "no stored state below the requested one");
+    const double* src = slots[kb].p;
+    if (scratch_step > base && scratch_step <= t) {
+        if (scratch_step == t) return st_cur.p;
+        base = scratch_step;
+        src = st_cur.p;
+    }
+    for (int s = base + 1; s <= t; s++) {
+        newmark_step(src, st_next.p, s);
+        std::swap(st_cur.p, st_next.p);
+        src = st_cur.p;
+    }
+    scratch_step = t;
+    return st_cur.p;
+}
+
 void Context::forward() {
     TT_CHECK(initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
+    scratch_step = -1;
     const int T = prm.nsteps;
     const long nd3 = kStateVecs * g.nd;  // whole states
     for (size_t i = 1; i < slot_step.size(); i++) slot_step[i] = -1;
@@ -244,6 +276,7 @@ static_assert(Context::kFwdHist >= 1 && 2 + Context::kFwdHist <= 8, "forward gue
 
 void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
     TT_CHECK(forward_done, TT_ERR_STATE, "tt_forward must be called first");
+    scratch_step = -1;  // the backward sweep uses st_cur as its work space
     const int T = prm.nsteps;
     const long nd = g.nd, nn = g.nn;
     const double al = prm.dalpha, be = prm.dbeta;
