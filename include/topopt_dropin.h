@@ -8,7 +8,7 @@
  * linked: the few PETSc names these signatures use are supplied here with minimal
  * meaning -- Vec is a host array of doubles, DM a structured-grid descriptor that also
  * carries the GPU context of its mesh, and the options database holds the command line.
- * MPIIO output objects can be created and are accepted, but write nothing.
+ * MPIIO writes the reference's single-rank field files.
  *
  * Differences a caller can observe:
  *   - Filter supports the PDE filter only (filterT = 2, Filter.cc:60-119 / 122-206);
