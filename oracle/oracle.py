@@ -111,6 +111,11 @@ def lib():
             "or_mma_design_change": (ctypes.c_double, [ctypes.c_long, dp, dp]),
             "or_mma_kkt_residual": (None, [ctypes.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp]),
             "or_mma_get_state": (None, [ctypes.c_void_p, dp, dp, dp]),
+            "or_sommg_basis": (ctypes.c_int, [P, dp, ctypes.c_int, ctypes.c_double, dp, dp]),
+            "or_sommg_reduce": (None, [P, dp, ctypes.c_int, dp, dp, dp, dp, dp]),
+            "or_reduced_forward": (ctypes.c_int, [P, ctypes.c_int, dp, dp, dp, dp, dp]),
+            "or_reduced_expand": (None, [ctypes.c_long, ctypes.c_int, ctypes.c_int, dp, dp, dp]),
+            "or_reduced_objective_gradient": (ctypes.c_int, [P, dp, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp]),
             "or_design_loop": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_double, dp, dp, dp]),
@@ -347,6 +352,68 @@ def design_loop(p, rmin, volfrac, V0, maxit, projection=False, beta=0.1, eta=0.1
     if itr < 0:
         raise RuntimeError("oracle design loop: a solve did not converge")
     return x, xp, hist.reshape(maxit, 5)[:itr]
+
+
+# ---------------------------------------------------------------- SOMMG reduced-order model
+def sommg_basis(p, xphys, nb, omega0):
+    """Basis Q (nb x nd, orthonormal) and Gram-Schmidt factor Ut (SOMMG.cc:40-230)."""
+    nd = 3 * nn(p)
+    Q = np.zeros((nb, nd))
+    Ut = np.zeros((nb, nb))
+    if lib().or_sommg_basis(ctypes.byref(p), _p(_f64(xphys)), nb, omega0, _p(Q), _p(Ut)) != 0:
+        raise RuntimeError("oracle SOMMG basis: CG did not converge")
+    return Q, Ut
+
+
+def sommg_reduce(p, xphys, Q):
+    """Reduced Kr, Mr, Cr (nb x nb) and load Pr (SOMMG.cc:236-262)."""
+    Q = _f64(Q)
+    nb = Q.shape[0]
+    Kr, Mr, Cr = (np.zeros((nb, nb)) for _ in range(3))
+    Pr = np.zeros(nb)
+    lib().or_sommg_reduce(ctypes.byref(p), _p(_f64(xphys)), nb, _p(Q), _p(Kr), _p(Mr), _p(Cr), _p(Pr))
+    return Kr, Mr, Cr, Pr
+
+
+def reduced_forward(p, Kr, Mr, Cr, Pr):
+    """Reduced Newmark states (nsteps+1, 3, nb)."""
+    nb = len(Pr)
+    rs = np.zeros((p.nsteps + 1) * 3 * nb)
+    if lib().or_reduced_forward(ctypes.byref(p), nb, _p(_f64(Kr)), _p(_f64(Mr)), _p(_f64(Cr)), _p(_f64(Pr)),
+                                _p(rs)) != 0:
+        raise RuntimeError("oracle reduced forward: singular system")
+    return rs.reshape(p.nsteps + 1, 3, nb)
+
+
+def reduced_expand(p, Q, rstates):
+    """Full states Q r (nsteps+1, 3, nd)."""
+    Q = _f64(Q)
+    nb, nd = Q.shape
+    full = np.zeros((p.nsteps + 1) * 3 * nd)
+    lib().or_reduced_expand(nd, nb, p.nsteps, _p(Q), _p(_f64(rstates)), _p(full))
+    return full.reshape(p.nsteps + 1, 3, nd)
+
+
+def reduced_objective_gradient(p, xphys, Q, Kr, Mr, Cr, rstates):
+    """Objective and sensitivities of the reduced model with the basis held fixed."""
+    Q = _f64(Q)
+    nb = Q.shape[0]
+    fx = ctypes.c_double()
+    df = np.zeros(nel(p))
+    if lib().or_reduced_objective_gradient(ctypes.byref(p), _p(_f64(xphys)), nb, _p(Q), _p(_f64(Kr)),
+                                           _p(_f64(Mr)), _p(_f64(Cr)), _p(_f64(rstates)), ctypes.byref(fx),
+                                           _p(df)) != 0:
+        raise RuntimeError("oracle reduced adjoint: singular system")
+    return fx.value, df
+
+
+def reduced_model(p, xphys, nb, omega0):
+    """The whole reduced path for one design: basis, reduced matrices, states, objective, gradient."""
+    Q, Ut = sommg_basis(p, xphys, nb, omega0)
+    Kr, Mr, Cr, Pr = sommg_reduce(p, xphys, Q)
+    rs = reduced_forward(p, Kr, Mr, Cr, Pr)
+    fx, df = reduced_objective_gradient(p, xphys, Q, Kr, Mr, Cr, rs)
+    return dict(Q=Q, Ut=Ut, Kr=Kr, Mr=Mr, Cr=Cr, Pr=Pr, rstates=rs, fx=fx, dfdx=df)
 
 
 # ---------------------------------------------------------------- assembled forms (scipy checks)

@@ -1341,3 +1341,297 @@ int or_design_loop(const or_params* p, double rmin, double volfrac, double V0, i
     free(xt); free(xold); free(df); free(dg); free(xmin); free(xmax); free(states);
     return status < 0 ? status : itr;
 }
+
+/* ------------------------------------------------------------------ */
+/* SOMMG reduced-order model (-reduction): the basis of                 */
+/* SOMMG::ConstructBasis (SOMMG.cc:40-265) and the reduced branches of  */
+/* LinearElasticity.cc (:404-461, 994-1003, 1015-1019, 1089-1107,       */
+/* 1683-1764, 2318-2335), where K, M, C, the load and the state-side    */
+/* objective gradients become their projections on the basis, the      */
+/* reduced systems are solved by LU (SetUpSolver2, :3007-3014), and the */
+/* objective and sensitivities use the states and multipliers projected */
+/* back to the full mesh.                                               */
+/* ------------------------------------------------------------------ */
+/* dense Gaussian elimination with partial pivoting: x = A^-1 b (A, n x n row-major, not modified) */
+static int dense_solve(int n, const double* A, const double* b, double* x) {
+    double* w = malloc(sizeof(double) * (size_t)n * (n + 1));
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) w[i * (n + 1) + j] = A[i * n + j];
+        w[i * (n + 1) + n] = b[i];
+    }
+    int ok = 1;
+    for (int k = 0; k < n; k++) {
+        int piv = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(w[i * (n + 1) + k]) > fabs(w[piv * (n + 1) + k])) piv = i;
+        if (w[piv * (n + 1) + k] == 0.0) { ok = 0; break; }
+        if (piv != k)
+            for (int j = 0; j <= n; j++) {
+                double t = w[k * (n + 1) + j];
+                w[k * (n + 1) + j] = w[piv * (n + 1) + j];
+                w[piv * (n + 1) + j] = t;
+            }
+        for (int i = k + 1; i < n; i++) {
+            double f = w[i * (n + 1) + k] / w[k * (n + 1) + k];
+            for (int j = k; j <= n; j++) w[i * (n + 1) + j] -= f * w[k * (n + 1) + j];
+        }
+    }
+    if (ok)
+        for (int i = n - 1; i >= 0; i--) {
+            double s = w[i * (n + 1) + n];
+            for (int j = i + 1; j < n; j++) s -= w[i * (n + 1) + j] * x[j];
+            x[i] = s / w[i * (n + 1) + i];
+        }
+    free(w);
+    return ok ? 0 : -1;
+}
+
+/* y = X w for a dense n x n X */
+static void dense_mv(int n, const double* X, const double* w, double* y) {
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += X[i * n + j] * w[j];
+        y[i] = s;
+    }
+}
+
+/* full = sum_k r[k] Q_k */
+static void expand(long nd, int nb, const double* Q, const double* r, double* full) {
+    memset(full, 0, sizeof(double) * nd);
+    for (int k = 0; k < nb; k++)
+        for (long i = 0; i < nd; i++) full[i] += r[k] * Q[(size_t)k * nd + i];
+}
+
+/* SOMMG::ConstructBasis (SOMMG.cc:40-230): Q_0 ~ P0^-1 N.F with P0 = K - om^2 M on the free dofs,
+   then Qb_j = P0^-1 N.(2 om M Q_{j-1}) orthonormalised against Q_0..Q_{j-1} by modified Gram-Schmidt
+   (:206-218).  The moment-matching correction of :136-187 enters as fvec += (-1.0)*(-1,0)*massVec
+   (:176): the comma expression (-1,0) is 0, so fvec is unchanged and the correction is not formed
+   here.  Q: nb vectors of nd; Ut: the nb x nb factor Utemp (row-major, upper triangle). */
+int or_sommg_basis(const or_params* p, const double* xphys, int nb, double om, double* Q, double* Ut) {
+    ctx_t c;
+    ctx_init(&c, p, xphys);
+    long nd = c.nd;
+    double* N = malloc(sizeof(double) * nd);
+    double* f = malloc(sizeof(double) * nd);
+    double* qb = malloc(sizeof(double) * nd);
+    or_bc_mask(p, N);
+    or_load_location(p, f);
+    for (long i = 0; i < nd; i++) f[i] *= N[i]; /* :77 */
+    memset(Ut, 0, sizeof(double) * nb * nb);
+    int status = 0;
+    for (int j = 0; j < nb; j++) {
+        if (j > 0) {
+            ctx_apply(&c, 0.0, 1.0, Q + (size_t)(j - 1) * nd, f); /* fvec = -(-2 om) M Q_{j-1} (:129-131) */
+            for (long i = 0; i < nd; i++) f[i] = N[i] * (f[i] * (-1.0 * (-2.0 * om))); /* N.fvec (:192) */
+        }
+        memset(qb, 0, sizeof(double) * nd);
+        if (ctx_pcg(&c, 1.0, -om * om, N, f, qb, p->rtol, p->maxit, NULL) < 0) status = -1;
+        for (int i = 0; i < j; i++) { /* :206-213 */
+            double d = vdot(nd, Q + (size_t)i * nd, qb);
+            Ut[i * nb + j] = d;
+            for (long h = 0; h < nd; h++) qb[h] -= d * Q[(size_t)i * nd + h];
+        }
+        double nrm = sqrt(vdot(nd, qb, qb)); /* :216-218 */
+        Ut[j * nb + j] = nrm;
+        for (long h = 0; h < nd; h++) Q[(size_t)j * nd + h] = qb[h] * (1.0 / nrm);
+    }
+    free(N); free(f); free(qb);
+    ctx_free(&c);
+    return status;
+}
+
+/* reduced matrices and load (SOMMG.cc:236-262): Kr_ij = Q_i.(K Q_j), Mr, Cr with C = dalpha M + dbeta K
+   (AssembleDampingMatrix, LinearElasticity.cc:2605-2619), Pr_i = (N.F).Q_i */
+void or_sommg_reduce(const or_params* p, const double* xphys, int nb, const double* Q, double* Kr, double* Mr,
+                     double* Cr, double* Pr) {
+    ctx_t c;
+    ctx_init(&c, p, xphys);
+    long nd = c.nd;
+    double* N = malloc(sizeof(double) * nd);
+    double* f = malloc(sizeof(double) * nd);
+    double* t = malloc(sizeof(double) * nd);
+    or_bc_mask(p, N);
+    or_load_location(p, f);
+    for (long i = 0; i < nd; i++) f[i] *= N[i];
+    for (int i = 0; i < nb; i++) Pr[i] = vdot(nd, f, Q + (size_t)i * nd);
+    for (int j = 0; j < nb; j++) {
+        const double* qj = Q + (size_t)j * nd;
+        ctx_apply(&c, 1.0, 0.0, qj, t);
+        for (int i = 0; i < nb; i++) Kr[i * nb + j] = vdot(nd, t, Q + (size_t)i * nd);
+        ctx_apply(&c, 0.0, 1.0, qj, t);
+        for (int i = 0; i < nb; i++) Mr[i * nb + j] = vdot(nd, t, Q + (size_t)i * nd);
+        ctx_apply(&c, p->dbeta, p->dalpha, qj, t);
+        for (int i = 0; i < nb; i++) Cr[i * nb + j] = vdot(nd, t, Q + (size_t)i * nd);
+    }
+    free(N); free(f); free(t);
+    ctx_free(&c);
+}
+
+/* reduced Newmark (CompleteInitialState :1015-1078 and SolveStateBlock :1298-1397 on the reduced
+   matrices, N = 1): rstates[(t*3 + {0,1,2})*nb + k] */
+int or_reduced_forward(const or_params* p, int nb, const double* Kr, const double* Mr, const double* Cr,
+                       const double* Pr, double* rstates) {
+    double a[6];
+    double dt = p->t1 / p->nsteps;
+    or_newmark_consts(p->ti, dt, a);
+    size_t n2 = (size_t)nb * nb;
+    double* S = malloc(sizeof(double) * n2);
+    double* B1 = malloc(sizeof(double) * n2);
+    double* B2 = malloc(sizeof(double) * n2);
+    double* B3 = malloc(sizeof(double) * n2);
+    double* R = malloc(sizeof(double) * nb);
+    double* w = malloc(sizeof(double) * nb);
+    for (size_t i = 0; i < n2; i++) {
+        S[i] = a[5] * Mr[i] + a[2] * Cr[i] + Kr[i]; /* -A (:1021-1024) */
+        B1[i] = a[5] * Mr[i] + a[2] * Cr[i];
+        B2[i] = a[3] * Mr[i] - a[0] * Cr[i];
+        B3[i] = a[4] * Mr[i] - a[1] * Cr[i];
+    }
+    int status = 0;
+    memset(rstates, 0, sizeof(double) * 3 * nb); /* U0 = DU0 = 0 */
+    double s0 = or_load_intensity(p, 0);
+    for (int i = 0; i < nb; i++) R[i] = Pr[i] * s0; /* R = RHS - C DU0 - K U0 with U0 = DU0 = 0 */
+    if (dense_solve(nb, Mr, R, rstates + 2 * nb)) status = -1;
+    for (int t = 1; t <= p->nsteps; t++) {
+        const double* up = rstates + (size_t)(t - 1) * 3 * nb;
+        const double* vp = up + nb;
+        const double* ap = up + 2 * nb;
+        double* un = rstates + (size_t)t * 3 * nb;
+        double* vn = un + nb;
+        double* an = un + 2 * nb;
+        double s = or_load_intensity(p, t);
+        for (int i = 0; i < nb; i++) R[i] = Pr[i] * s;
+        dense_mv(nb, B1, up, w);
+        for (int i = 0; i < nb; i++) R[i] += w[i];
+        dense_mv(nb, B2, vp, w);
+        for (int i = 0; i < nb; i++) R[i] += w[i];
+        dense_mv(nb, B3, ap, w);
+        for (int i = 0; i < nb; i++) R[i] += w[i];
+        if (dense_solve(nb, S, R, un)) status = -1;
+        for (int i = 0; i < nb; i++) {
+            double du = un[i] - up[i];
+            vn[i] = a[0] * vp[i] + a[1] * ap[i] + a[2] * du;
+            an[i] = -a[3] * vp[i] - a[4] * ap[i] + a[5] * du;
+        }
+    }
+    free(S); free(B1); free(B2); free(B3); free(R); free(w);
+    return status;
+}
+
+/* full states Q r for every step: full[(t*3 + c)*nd + i] */
+void or_reduced_expand(long nd, int nb, int nsteps, const double* Q, const double* rstates, double* full) {
+    for (int t = 0; t <= nsteps; t++)
+        for (int c = 0; c < 3; c++)
+            expand(nd, nb, Q, rstates + ((size_t)t * 3 + c) * nb, full + ((size_t)t * 3 + c) * nd);
+}
+
+/* objective and sensitivities of the reduced model: the adjoint recurrences of SolveLagrangen /
+   SolveLagrange0 (:678-954) on the reduced matrices with the projected state gradients
+   Q^T hdu, Q^T hddu, Q^T hdddu (:2318-2335); the multipliers are expanded as Q lambda (:1750-1764,
+   1850-1863) and enter the full-mesh sensitivity loops with the expanded states (the basis is held
+   fixed: its dependence on the design is not differentiated, as in the reference). */
+int or_reduced_objective_gradient(const or_params* p, const double* xphys, int nb, const double* Q,
+                                  const double* Kr, const double* Mr, const double* Cr, const double* rstates,
+                                  double* fx, double* dfdx) {
+    ctx_t c;
+    ctx_init(&c, p, xphys);
+    long nd = c.nd, nel = c.nel;
+    const double* a = c.a;
+    int T = p->nsteps, status = 0;
+    size_t n2 = (size_t)nb * nb;
+    double* full = malloc(sizeof(double) * (size_t)(T + 1) * 3 * nd);
+    or_reduced_expand(nd, nb, T, Q, rstates, full);
+    double* S = malloc(sizeof(double) * n2);
+    double* B1 = malloc(sizeof(double) * n2);
+    double* B2 = malloc(sizeof(double) * n2);
+    double* B3 = malloc(sizeof(double) * n2);
+    for (size_t i = 0; i < n2; i++) {
+        S[i] = a[5] * Mr[i] + a[2] * Cr[i] + Kr[i];
+        B1[i] = a[5] * Mr[i] + a[2] * Cr[i];
+        B2[i] = a[3] * Mr[i] - a[0] * Cr[i];
+        B3[i] = a[4] * Mr[i] - a[1] * Cr[i];
+    }
+    double* hdu = malloc(sizeof(double) * nd);
+    double* hddu = malloc(sizeof(double) * nd);
+    double* hdddu = malloc(sizeof(double) * nd);
+    double* lamf = malloc(sizeof(double) * nd);
+    double* h1 = malloc(sizeof(double) * nb);
+    double* h2 = malloc(sizeof(double) * nb);
+    double* h3 = malloc(sizeof(double) * nb);
+    double* lam = calloc((size_t)(T + 2) * nb, sizeof(double));
+    double* mu = malloc(sizeof(double) * nb);
+    double* nu = malloc(sizeof(double) * nb);
+    double* mun = calloc(nb, sizeof(double));
+    double* nun = calloc(nb, sizeof(double));
+    double* b1 = malloc(sizeof(double) * nb);
+    double* b2 = malloc(sizeof(double) * nb);
+    double* b3 = malloc(sizeof(double) * nb);
+    double* R1 = malloc(sizeof(double) * nb);
+    double* nu0 = malloc(sizeof(double) * nb);
+#define PROJECT(s)                                                                        \
+    do {                                                                                  \
+        grad_obj_state(&c, (s), (s) + nd, (s) + 2 * nd, hdu, hddu, hdddu);                \
+        for (int k = 0; k < nb; k++) {                                                    \
+            h1[k] = vdot(nd, hdu, Q + (size_t)k * nd);                                    \
+            h2[k] = vdot(nd, hddu, Q + (size_t)k * nd);                                   \
+            h3[k] = vdot(nd, hdddu, Q + (size_t)k * nd);                                  \
+        }                                                                                 \
+    } while (0)
+    for (int t = T; t >= 1; t--) {
+        PROJECT(full + (size_t)t * 3 * nd);
+        double* lt = lam + (size_t)t * nb;
+        if (t == T) {
+            for (int i = 0; i < nb; i++) {
+                mu[i] = -h2[i];
+                nu[i] = -h3[i];
+                R1[i] = -(h1[i] - a[2] * mu[i] - a[5] * nu[i]);
+            }
+        } else {
+            const double* ln = lam + (size_t)(t + 1) * nb;
+            dense_mv(nb, B1, ln, b1);
+            dense_mv(nb, B2, ln, b2);
+            dense_mv(nb, B3, ln, b3);
+            for (int i = 0; i < nb; i++) {
+                double m = h2[i] + b2[i] - a[0] * mun[i] + a[3] * nun[i];
+                double n_ = h3[i] + b3[i] + a[4] * nun[i];
+                double r = -(h1[i] + b1[i] + a[2] * mun[i] + a[5] * nun[i]);
+                mu[i] = -m;
+                nu[i] = -n_;
+                R1[i] = r + a[2] * mu[i] + a[5] * nu[i];
+            }
+        }
+        /* A lambda = R1 with A = -S (:1118-1125) */
+        for (int i = 0; i < nb; i++) R1[i] = -R1[i];
+        if (dense_solve(nb, S, R1, lt)) status = -1;
+        memcpy(mun, mu, sizeof(double) * nb);
+        memcpy(nun, nu, sizeof(double) * nb);
+    }
+    {
+        PROJECT(full);
+        dense_mv(nb, B3, lam + (size_t)1 * nb, b3);
+        for (int i = 0; i < nb; i++) R1[i] = -(h3[i] + b3[i] - a[1] * mun[i] + a[4] * nun[i]);
+        if (dense_solve(nb, Mr, R1, nu0)) status = -1; /* dRdS = M (:763-769) */
+    }
+#undef PROJECT
+    double f = 0.0;
+    for (int t = 1; t <= T; t++) {
+        const double* s = full + (size_t)t * 3 * nd;
+        f += objective_state(&c, s, s + nd, s + 2 * nd);
+    }
+    f += objective_state(&c, full, full + nd, full + 2 * nd);
+    memset(dfdx, 0, sizeof(double) * nel);
+    for (int t = 1; t <= T; t++) {
+        const double* s = full + (size_t)t * 3 * nd;
+        const double* sb = full + (size_t)(t - 1) * 3 * nd;
+        expand(nd, nb, Q, lam + (size_t)t * nb, lamf);
+        sens_n(&c, lamf, s, s + nd, s + 2 * nd, sb, sb + nd, sb + 2 * nd, dfdx);
+    }
+    expand(nd, nb, Q, nu0, lamf);
+    sens_0(&c, lamf, full, full + nd, full + 2 * nd, dfdx);
+    *fx = f;
+    free(full); free(S); free(B1); free(B2); free(B3); free(hdu); free(hddu); free(hdddu); free(lamf);
+    free(h1); free(h2); free(h3); free(lam); free(mu); free(nu); free(mun); free(nun); free(b1); free(b2);
+    free(b3); free(R1); free(nu0);
+    ctx_free(&c);
+    return status;
+}

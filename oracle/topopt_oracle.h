@@ -113,6 +113,19 @@ void or_mma_kkt_residual(or_mma* mma, const double* x, const double* dfdx, const
                          double* normInf);
 void or_mma_get_state(or_mma* mma, double* L, double* U, double* lam);
 
+/* SOMMG reduced-order model (SOMMG.cc:40-265; LinearElasticity.cc reduced branches).
+   Q: nb vectors of nd (row-major); Ut: nb x nb Gram-Schmidt factor; Kr, Mr, Cr: nb x nb; Pr: nb;
+   rstates[(t*3 + {0,1,2})*nb + k] for t = 0..nsteps */
+int or_sommg_basis(const or_params* p, const double* xphys, int nb, double om, double* Q, double* Ut);
+void or_sommg_reduce(const or_params* p, const double* xphys, int nb, const double* Q, double* Kr, double* Mr,
+                     double* Cr, double* Pr);
+int or_reduced_forward(const or_params* p, int nb, const double* Kr, const double* Mr, const double* Cr,
+                       const double* Pr, double* rstates);
+void or_reduced_expand(long nd, int nb, int nsteps, const double* Q, const double* rstates, double* full);
+int or_reduced_objective_gradient(const or_params* p, const double* xphys, int nb, const double* Q,
+                                  const double* Kr, const double* Mr, const double* Cr, const double* rstates,
+                                  double* fx, double* dfdx);
+
 /* main.cc optimisation loop: runs maxit design iterations from x0 (V0).
    hist (maxit*5): fx_true, fx_scaled, gx, ch, mnd per iteration.  x/xphys out. */
 int or_design_loop(const or_params* p, double rmin, double volfrac, double V0, int maxit, int projection,
