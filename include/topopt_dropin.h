@@ -16,7 +16,8 @@
  *   - restart files (Restart0x.dat, Restart0x_itr_f0.dat, RestartSol0x.dat) are written and read in
  *     PETSc's binary Vec format with the reference's options; the cp%04d.dat checkpoint files are
  *     not written (the binomial checkpoints stay in device memory), and -solutionAsU0 is rejected;
- *   - the reduced-order path (reduction = TRUE) is rejected;
+ *   - the reduced-order path (reduction = TRUE, SOMMG) runs with the reduced systems solved directly,
+ *     as in the reference; it reads and writes no solution restart file (RestartSol0x.dat);
  *   - numerical non-convergence and non-finite values are reported as errors (< 0).
  */
 #ifndef TOPOPT_DROPIN_H
@@ -176,6 +177,7 @@ class LinearElasticity {
 
    private:
     PetscInt Ts, ncp;
+    PetscBool reduced;  // SOMMG reduced-order model (-reduction)
     tt_ctx* ctx;
     bool forward_done;
     PetscErrorCode WriteStates(MPIIO* output, Vec xPhys);

@@ -70,6 +70,12 @@ typedef struct tt_params {
                                       captured HIP graph (bit-identical results; default 0: measured
                                       slower than eager launches on ROCm 7 for this ~30-kernel
                                       iteration) */
+    int reduction;                 /* 1: SOMMG reduced-order model (-reduction): the Newmark and adjoint
+                                      sweeps run on the projection onto `basis` moment-matching vectors
+                                      (SOMMG.cc:40-265; LinearElasticity.cc:404-461, 994-1003) */
+    int basis;                     /* basis vectors (-basis, 1..64) */
+    double omega0;                 /* expansion frequency of the basis solves with K - omega0^2 M (-omega0,
+                                      passed unconverted as in LinearElasticity.cc:996) */
 } tt_params;
 
 typedef struct tt_stats {
@@ -125,7 +131,8 @@ int tt_forward(tt_ctx* ctx);
 
 /* State at step t (0..nsteps) of the last tt_forward (GetStateFieldU/DU/DDU, LinearElasticity.h:68-78);
    3*nn doubles each (may be NULL).  A state that the binomial checkpoints did not keep is recomputed
-   from the nearest stored one; asking for t = 1, 2, ... in turn costs one Newmark step each. */
+   from the nearest stored one; asking for t = 1, 2, ... in turn costs one Newmark step each.  With the
+   reduced-order model the state is the reduced one expanded on the basis. */
 int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device);
 
 /* ComputeObjectiveSensitivities (LinearElasticity.cc:1625-1908): objective fx and dfdx
@@ -176,9 +183,20 @@ int tt_mma_set_state(tt_mma* mma, int k, const double* xo1, const double* xo2, c
    multiplier lambda_T (LT) and the t = 0 adjoint multiplier nu0 (DDL0), each nd values in host
    memory.  get returns the latest ones (zeros before the first solve); set makes them the starting
    guesses of the next tt_complete_initial_state (a0) and tt_objective_gradient (lambda_T, nu0), as the
-   reference's persistent vectors are.  A null pointer leaves that vector as it is. */
+   reference's persistent vectors are.  A null pointer leaves that vector as it is.
+   With the reduced-order model (reduction = 1) the reduced systems are solved directly, so there are
+   no starting guesses: both return TT_ERR_STATE. */
 int tt_get_restart_vectors(tt_ctx* ctx, double* a0, double* lamT, double* nu0);
 int tt_set_restart_vectors(tt_ctx* ctx, const double* a0, const double* lamT, const double* nu0);
+
+/* The SOMMG reduced model (reduction = 1) of the last tt_complete_initial_state: the members QQ, Kr, Mr,
+   Cr, Pr of the reference's SOMMG object (SOMMG.h:16-22) and the Gram-Schmidt factor Utemp of
+   ConstructBasis (SOMMG.cc:48-218), plus the reduced states U, DU, DDU of the last tt_forward.
+   Q: basis x nd (on_device selects host or device memory); Ut, Kr, Mr, Cr: basis x basis row-major;
+   Pr: basis; rstates: (nsteps + 1) x 3 x basis (u, v, a per step), host memory.  Any pointer may be
+   NULL.  TT_ERR_STATE without the reduced model, or for rstates before tt_forward. */
+int tt_get_reduced_model(tt_ctx* ctx, double* Q, int on_device, double* Ut, double* Kr, double* Mr, double* Cr,
+                         double* Pr, double* rstates);
 
 /* PETSc binary vector files, as PetscViewerBinary VecView / VecLoad write and read them for the
    reference's checkpoint and restart files (LinearElasticity.cc:1128-1137, 1261-1276, 2474-2486;

@@ -27,7 +27,7 @@ EXPORTED = [
     "tt_mma_kkt_residual", "tt_mma_get_asymptotes", "tt_get_stats", "tt_reset_stats", "tt_synchronize",
     "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
     "tt_mma_get_state", "tt_mma_set_state", "tt_get_restart_vectors", "tt_set_restart_vectors",
-    "tt_petsc_write_vecs", "tt_petsc_read_vecs",
+    "tt_petsc_write_vecs", "tt_petsc_read_vecs", "tt_get_reduced_model",
 ]
 
 
@@ -46,6 +46,7 @@ class Params(ctypes.Structure):
         ("cheb_degree", ctypes.c_int), ("checkpoint_slots", ctypes.c_int),
         ("rmin", ctypes.c_double), ("filter_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
         ("graphs", ctypes.c_int),
+        ("reduction", ctypes.c_int), ("basis", ctypes.c_int), ("omega0", ctypes.c_double),
     ]
 
 
@@ -102,6 +103,7 @@ def lib():
             "tt_get_restart_vectors": [vp, vp, vp, vp], "tt_set_restart_vectors": [vp, vp, vp, vp],
             "tt_petsc_write_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
             "tt_petsc_read_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
+            "tt_get_reduced_model": [vp, vp, ip, vp, vp, vp, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -259,6 +261,22 @@ class Context:
         out = [np.zeros(self.nd) for _ in range(3)]
         self._chk(lib().tt_get_restart_vectors(self.h, *[o.ctypes.data for o in out]))
         return tuple(out)
+
+    def reduced_model(self, with_states=False):
+        """The SOMMG reduced model (reduction = 1): dict of Q (basis x nd), Ut, Kr, Mr, Cr (basis x basis),
+        Pr (basis) and, with_states, the reduced states (nsteps + 1, 3, basis) of the last forward sweep."""
+        nb = self.params.basis
+        Q = np.zeros((nb, self.nd))
+        Ut, Kr, Mr, Cr = (np.zeros((nb, nb)) for _ in range(4))
+        Pr = np.zeros(nb)
+        rs = np.zeros((self.params.nsteps + 1, 3, nb)) if with_states else None
+        self._chk(lib().tt_get_reduced_model(self.h, Q.ctypes.data, 0, Ut.ctypes.data, Kr.ctypes.data,
+                                             Mr.ctypes.data, Cr.ctypes.data, Pr.ctypes.data,
+                                             None if rs is None else rs.ctypes.data))
+        out = dict(Q=Q, Ut=Ut, Kr=Kr, Mr=Mr, Cr=Cr, Pr=Pr)
+        if with_states:
+            out["rstates"] = rs
+        return out
 
     def set_restart_vectors(self, a0=None, lamT=None, nu0=None):
         arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (a0, lamT, nu0)]

@@ -574,8 +574,9 @@ PetscErrorCode TopOpt::WriteRestartFiles(PetscInt* itr, MMA* mma) {
 LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt nlvls_in, PetscInt ncp_in,
                                    PetscInt tsc_in, PetscScalar t1_in, PetscInt ti_in, PetscInt objective_in,
                                    PetscInt sInt_in, PetscInt mInt_in, PetscScalar t0_in, PetscInt boundaries_in,
-                                   PetscInt loadloc_in, PetscBool reduction_in, PetscInt, PetscScalar dalpha_in,
-                                   PetscScalar dbeta_in, PetscScalar, PetscInt optimizationRegion, PetscScalar) {
+                                   PetscInt loadloc_in, PetscBool reduction_in, PetscInt basis_in,
+                                   PetscScalar dalpha_in, PetscScalar dbeta_in, PetscScalar omega0_in,
+                                   PetscInt optimizationRegion, PetscScalar) {
     da_nodal = da_nodes;
     ctx = nullptr;
     forward_done = false;
@@ -583,12 +584,13 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     for (auto& v : mat) v = std::nan("");
     Ts = ncp_in * tsc_in + 1;  // LinearElasticity.cc:96
     ncp = ncp_in;
-    if (reduction_in) {
-        std::fprintf(stderr, "[topopt] the reduced-order (-reduction) path is not part of this build\n");
-        std::exit(1);
-    }
+    reduced = reduction_in;
     tt_params p;
     tt_default_params(&p);
+    // SOMMG reduced-order model (LinearElasticity.cc:404-461, 506-508, 994-1003)
+    p.reduction = reduction_in ? 1 : 0;
+    p.basis = basis_in;
+    p.omega0 = omega0_in;
     p.nelx = da_nodes->nxyz[0] - 1;
     p.nely = da_nodes->nxyz[1] - 1;
     p.nelz = da_nodes->nxyz[2] - 1;
@@ -638,7 +640,10 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     const std::string wd = option_string("-workdir", "./");
     filename00 = wd + "/RestartSol00.dat";
     filename01 = wd + "/RestartSol01.dat";
-    if (restart && !onlyDesign) {
+    if (restart && !onlyDesign && reduced) {
+        // the reduced systems are solved directly (SetUpSolver2, :3007-3014): no starting guesses to read
+        PetscPrintf(PETSC_COMM_WORLD, "# Reduced-order model: -restartFileVecSol is not read (direct reduced solves)\n");
+    } else if (restart && !onlyDesign) {
         const std::string fileVec = option_string("-restartFileVecSol", "");
         PetscPrintf(PETSC_COMM_WORLD, "# Restarting with solution (State Vector) from (-restartFileVecSol): %s \n",
                     fileVec.c_str());
@@ -746,6 +751,12 @@ PetscErrorCode LinearElasticity::VolumeConstraint(PetscScalar* gx, Vec dgdx, Vec
 // LinearElasticity.cc:2457-2496: DDU0, LT, DDL0 (PETSc binary), alternating between the 00 and 01 files
 PetscErrorCode LinearElasticity::WriteRestartFiles() {
     if (!restart) return -1;
+    if (reduced) {
+        // the reference writes the reduced DDU0, LT, DDL0 here, starting guesses of direct solves that
+        // need none; this build writes no solution restart file with the reduced-order model
+        PetscPrintf(PETSC_COMM_WORLD, "# Reduced-order model: no solution restart file\n");
+        return 0;
+    }
     flip = flip ? PETSC_FALSE : PETSC_TRUE;
     const long nd = 3 * tt_num_nodes(ctx);
     std::vector<double> a0(nd), lt(nd), dl0(nd);

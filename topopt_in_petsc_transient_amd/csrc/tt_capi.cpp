@@ -20,6 +20,9 @@ static void check_params(const tt_params& p) {
     TT_CHECK(p.cheb_degree >= 1 && p.cheb_degree <= 16, TT_ERR_ARG, "cheb_degree must be 1..16");
     TT_CHECK(p.rtol > 0.0 && p.maxit > 0, TT_ERR_ARG, "need rtol > 0 and maxit > 0");
     TT_CHECK(p.loadsignal != 1 || p.pulse_steps > 0, TT_ERR_ARG, "pulse_steps must be positive");
+    TT_CHECK(!p.reduction || (p.basis >= 1 && p.basis <= 64), TT_ERR_ARG, "basis must be 1..64");
+    TT_CHECK(!p.reduction || p.omega0 != 0.0, TT_ERR_ARG,
+             "omega0 must be nonzero: the basis vectors after the first are solves with 2 omega0 M Q");
     long nn = (long)(p.nelx + 1) * (p.nely + 1) * (p.nelz + 1);
     TT_CHECK(3 * nn < (1L << 31), TT_ERR_ARG, "too many dofs for 32-bit indexing");
     // the kernels address FP64 dof vectors through 32-bit byte offsets and form node indices with
@@ -166,6 +169,14 @@ Context::Context(const tt_params& p) : prm(p) {
     // state storage: every state, or s binomial slots + the t = 0 state
     const int T = p.nsteps;
     int nslots = (p.checkpoint_slots <= 0 || p.checkpoint_slots >= T) ? T + 1 : p.checkpoint_slots + 1;
+    reduced = p.reduction != 0;
+    if (reduced) {
+        // reduced-order model: the states live on the host in basis coordinates; the full-mesh
+        // states and multipliers are expanded into st_cur / st_next / lam when needed
+        nslots = 1;
+        nbasis = p.basis;
+        Qb.alloc((size_t)nbasis * nd);
+    }
     store_all = (nslots == T + 1);
     size_t freeb = 0, totalb = 0;
     TT_HIP(hipMemGetInfo(&freeb, &totalb));
@@ -178,7 +189,7 @@ Context::Context(const tt_params& p) : prm(p) {
     slot_step.assign(nslots, -1);
     st_cur.alloc((size_t)kStateVecs * nd);
     st_next.alloc((size_t)kStateVecs * nd);
-    if (!store_all) {
+    if (!store_all && !reduced) {
         st_last.alloc((size_t)kStateVecs * nd);
         st_hi.alloc((size_t)kStateVecs * nd);
     }
@@ -418,6 +429,7 @@ int tt_default_params(tt_params* p) {
     p->rmin = 0.08; p->filter_rtol = 1e-10;
     p->verbose = 0;
     p->graphs = 0;
+    p->reduction = 0; p->basis = 5; p->omega0 = 1.0;  // TopOpt.cc:157-159
     return TT_OK;
 }
 
@@ -770,8 +782,29 @@ int tt_mma_set_state(tt_mma* m, int k, const double* xo1, const double* xo2, con
     TT_API_END
 }
 
+int tt_get_reduced_model(tt_ctx* ctx, double* Q, int on_device, double* Ut, double* Kr, double* Mr, double* Cr,
+                         double* Pr, double* rstates) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(C.reduced, TT_ERR_STATE, "the context was created without the reduced-order model");
+    TT_CHECK(C.initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
+    TT_CHECK(!rstates || C.forward_done, TT_ERR_STATE, "tt_forward must be called first");
+    const size_t nb = C.nbasis, n2 = nb * nb;
+    if (Q)
+        TT_HIP(hipMemcpyAsync(Q, C.Qb.p, nb * C.g.nd * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              C.stream));
+    if (Ut) std::memcpy(Ut, C.Ut.data(), n2 * 8);
+    if (Kr) std::memcpy(Kr, C.Kr.data(), n2 * 8);
+    if (Mr) std::memcpy(Mr, C.Mr.data(), n2 * 8);
+    if (Cr) std::memcpy(Cr, C.Cr.data(), n2 * 8);
+    if (Pr) std::memcpy(Pr, C.Pr.data(), nb * 8);
+    if (rstates) std::memcpy(rstates, C.rst.data(), C.rst.size() * 8);
+    C.sync();
+    TT_API_END
+}
+
 int tt_get_restart_vectors(tt_ctx* ctx, double* a0, double* lamT, double* nu0) {
     TT_API_BEGIN(ctx)
+    TT_CHECK(!C.reduced, TT_ERR_STATE, "no restart vectors with the reduced-order model (direct reduced solves)");
     const size_t bytes = C.g.nd * sizeof(double);
     if (a0) TT_HIP(hipMemcpyAsync(a0, C.slots[0].p + 2 * C.g.nd, bytes, hipMemcpyDeviceToHost, C.stream));
     if (lamT) TT_HIP(hipMemcpyAsync(lamT, C.lamT.p, bytes, hipMemcpyDeviceToHost, C.stream));
@@ -782,6 +815,7 @@ int tt_get_restart_vectors(tt_ctx* ctx, double* a0, double* lamT, double* nu0) {
 
 int tt_set_restart_vectors(tt_ctx* ctx, const double* a0, const double* lamT, const double* nu0) {
     TT_API_BEGIN(ctx)
+    TT_CHECK(!C.reduced, TT_ERR_STATE, "no restart vectors with the reduced-order model (direct reduced solves)");
     const size_t bytes = C.g.nd * sizeof(double);
     if (a0) {
         TT_HIP(hipMemcpyAsync(C.slots[0].p + 2 * C.g.nd, a0, bytes, hipMemcpyHostToDevice, C.stream));

@@ -1,6 +1,7 @@
 // Library context: device data of one transient-elasticity problem on one GPU.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -145,6 +146,21 @@ struct Context {
     double KF[64];
     DBuf<double> KFd, fdinv, fU, frhs, fr, fz, fp, fq, ftmp_e, ftmp_e2;
     Grid fg{};
+
+    // SOMMG reduced-order model (tt_params.reduction): basis Q (nbasis vectors of nd), its
+    // Gram-Schmidt factor, the reduced stiffness, mass, damping and load, and the reduced states
+    // rst[(t*3 + {0,1,2})*nbasis + k] (u, v, a) of the last forward sweep
+    bool reduced = false;
+    int nbasis = 0;
+    DBuf<double> Qb;
+    std::vector<double> Ut, Kr, Mr, Cr, Pr, rst;
+    void reduced_initial_state();             // basis, reduced matrices, a0 (CompleteInitialState)
+    void reduced_forward();                   // reduced Newmark sweep (SolveStateBlock)
+    void expand(const double* coef, double* out);  // out = sum_k coef[k] Q_k
+    void expand_state(int t, double* out);    // u, v, a of reduced state t on the full mesh
+    void project3(const double* h0, const double* h1, const double* h2, double* out);  // out[3k+i] = Q_k.h_i
+    void reduced_adjoint(const std::function<void(const double*)>& grad_state,
+                         const std::function<void(int, const double*, const double*, const double*, int)>& sens);
 
     // stats
     tt_stats stats{};

@@ -1293,6 +1293,26 @@ void launch_dot(const double* a, const double* b, long n, Reducer& red, int regi
     TT_HIP(hipGetLastError());
 }
 
+__global__ void k_dot3(const double* __restrict__ q, const double* __restrict__ a, const double* __restrict__ b,
+                       const double* __restrict__ c, long n, double* part) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double qi = q[i];
+        acc[0] = fma(qi, a[i], acc[0]);
+        acc[1] = fma(qi, b[i], acc[1]);
+        acc[2] = fma(qi, c[i], acc[2]);
+    }
+    write_partials<3>(acc, part);
+}
+void launch_dot3(const double* q, const double* a, const double* b, const double* c, long n, Reducer& red, int region,
+                 hipStream_t s) {
+    const int nb = rb_blocks(n);
+    hipLaunchKernelGGL(k_dot3, dim3(nb), dim3(BS), 0, s, q, a, b, c, n, red.part(region));
+    red.nblk[region] = nb;
+    red.nv[region] = 3;
+    TT_HIP(hipGetLastError());
+}
+
 // gamma = z.r, delta = z.r_prev (flexible CG, Polak-Ribiere form)
 __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ rp,
                            long n, double* part, int first) {

@@ -38,6 +38,12 @@ double Context::load_intensity(int step) const {
 // ------------------------------------------------------------------ forward
 void Context::complete_initial_state() {
     TT_CHECK(density_set, TT_ERR_STATE, "tt_set_density must be called first");
+    if (reduced) {
+        reduced_initial_state();
+        initial_done = true;
+        forward_done = false;
+        return;
+    }
     const long nd = g.nd;
     mg.setup_operator(cK_A, cM_A);
     launch_diag_inv(op_fine64, 0.0, 1.0, dinvM.p, true, stream);
@@ -207,6 +213,11 @@ struct Revolve {
 const double* Context::state_at(int t) {
     TT_CHECK(forward_done, TT_ERR_STATE, "tt_forward must be called first");
     TT_CHECK(t >= 0 && t <= prm.nsteps, TT_ERR_ARG, "time step out of range");
+    if (reduced) {  // the reduced state expanded on the full mesh
+        expand_state(t, st_cur.p);
+        scratch_step = -1;
+        return st_cur.p;
+    }
     int base = -1, kb = -1;
     for (size_t i = 0; i < slot_step.size(); i++) {
         if (slot_step[i] == t) return slots[i].p;
@@ -235,6 +246,11 @@ const double* Context::state_at(int t) {
 void Context::forward() {
     TT_CHECK(initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
     scratch_step = -1;
+    if (reduced) {
+        reduced_forward();
+        forward_done = true;
+        return;
+    }
     const int T = prm.nsteps;
     const long nd3 = kStateVecs * g.nd;  // whole states
     for (size_t i = 1; i < slot_step.size(); i++) slot_step[i] = -1;
@@ -378,46 +394,50 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
         if (t == T) launch_copy(lamh[0].p, lamT.p, nd, true, true, stream);
     };
 
-    launch_copy(lamT.p, lam.p, nd, true, true, stream);
-    if (store_all) {
-        for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
+    if (reduced) {
+        reduced_adjoint(grad_state, sens);
     } else {
-        // The adjoint step at t reads states t and t-1 only.  The sweep runs t = T, T-1, ..., 1, so
-        // state t is the state t-1 of the previous adjoint step: it is kept (in its checkpoint slot,
-        // or moved to st_hi) instead of being recomputed from state t-1, which would cost one Newmark
-        // step per adjoint step.  The schedule then needs the forward sweep plus the binomial
-        // recomputation only: 379 Newmark steps for 200 steps and 20 slots, against 579 when every
-        // adjoint step redoes its forward step.  Checkpoints are read in place, never restored.
-        const double* later = st_last.p;  // state T
-        int later_step = -1;              // the checkpointed step `later` points into, or -1
-        Revolve rv;
-        rv.c = this;
-        rv.adjoint_at = [&](int t, const double* prev) {
-            adjoint_step(t, later, prev);
-            if (prev == st_cur.p) {  // out of the way of the next recomputation
-                std::swap(st_hi.p, st_cur.p);
+        launch_copy(lamT.p, lam.p, nd, true, true, stream);
+        if (store_all) {
+            for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
+        } else {
+            // The adjoint step at t reads states t and t-1 only.  The sweep runs t = T, T-1, ..., 1, so
+            // state t is the state t-1 of the previous adjoint step: it is kept (in its checkpoint slot,
+            // or moved to st_hi) instead of being recomputed from state t-1, which would cost one Newmark
+            // step per adjoint step.  The schedule then needs the forward sweep plus the binomial
+            // recomputation only: 379 Newmark steps for 200 steps and 20 slots, against 579 when every
+            // adjoint step redoes its forward step.  Checkpoints are read in place, never restored.
+            const double* later = st_last.p;  // state T
+            int later_step = -1;              // the checkpointed step `later` points into, or -1
+            Revolve rv;
+            rv.c = this;
+            rv.adjoint_at = [&](int t, const double* prev) {
+                adjoint_step(t, later, prev);
+                if (prev == st_cur.p) {  // out of the way of the next recomputation
+                    std::swap(st_hi.p, st_cur.p);
+                    later = st_hi.p;
+                    later_step = -1;
+                } else {
+                    later = prev;  // a checkpoint slot, valid until it is released
+                    later_step = t - 1;
+                }
+            };
+            rv.on_release = [&](int step) {
+                if (step != later_step) return;
+                launch_copy(later, st_hi.p, 3 * nd, true, true, stream);  // u, v, a: what an adjoint step reads
                 later = st_hi.p;
                 later_step = -1;
-            } else {
-                later = prev;  // a checkpoint slot, valid until it is released
-                later_step = t - 1;
-            }
-        };
-        rv.on_release = [&](int step) {
-            if (step != later_step) return;
-            launch_copy(later, st_hi.p, 3 * nd, true, true, stream);  // u, v, a: what an adjoint step reads
-            later = st_hi.p;
-            later_step = -1;
-        };
-        rv.reverse(0, T, (int)slots.size() - 1);
+            };
+            rv.reverse(0, T, (int)slots.size() - 1);
+        }
+        // t = 0 (SolveLagrange0, :678-804; ComputeSensitivitiesElementLoop0, :2080-2204)
+        const double* s0 = slots[0].p;
+        grad_state(s0);
+        launch_apply_split(op_fine64, 1.0, 1.0, lamh[0].p, Kl.p, Ml.p, stream);
+        launch_adjoint0_rhs(Kl.p, Ml.p, hdddu.p, mun.p, nun.p, mask.p, coef, rhs.p, nn, stream);
+        solve_M(rhs.p, nu0.p);
+        sens(1, nu0.p, s0, nullptr, 0);
     }
-    // t = 0 (SolveLagrange0, :678-804; ComputeSensitivitiesElementLoop0, :2080-2204)
-    const double* s0 = slots[0].p;
-    grad_state(s0);
-    launch_apply_split(op_fine64, 1.0, 1.0, lamh[0].p, Kl.p, Ml.p, stream);
-    launch_adjoint0_rhs(Kl.p, Ml.p, hdddu.p, mun.p, nun.p, mask.p, coef, rhs.p, nn, stream);
-    solve_M(rhs.p, nu0.p);
-    sens(1, nu0.p, s0, nullptr, 0);
     // objective: sum over t = 1..T, then t = 0 (:1778-1802)
     std::vector<double> fp((size_t)(T + 1) * nbs);
     TT_HIP(hipMemcpyAsync(fp.data(), fxs.p, sizeof(double) * fp.size(), hipMemcpyDeviceToHost, stream));
