@@ -1,11 +1,16 @@
 """SOMMG reduced-order model (-reduction) on the GPU against the oracle: the basis, the Gram-Schmidt
 factor, the reduced matrices and load (SOMMG.cc:40-265), the reduced Newmark states, and the
 objective and sensitivities of the reduced model (the reduced branches of LinearElasticity.cc)."""
+import json
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from tests.gpu_helpers import problem, random_density, rel
+from tests.test_gpu_design_loop import ARGS, DRIVER
 from topopt_in_petsc_transient_amd import capi
 
 pytestmark = pytest.mark.gpu
@@ -120,3 +125,31 @@ def test_reduced_model_errors():
     with pytest.raises(capi.TTError):
         c.reduced_model()
     c.close()
+
+
+def test_drop_in_driver_runs_the_reduced_model(tmp_path):
+    """The main.cc driver with -reduction 1 -basis 5 -omega0 0.01 on the 32x16x16 design loop: the
+    objective of its first design (x = V0 = 0.12, PDE filter and projection) is the oracle's reduced one."""
+    assert os.path.exists(DRIVER), "build the driver first (__graft_entry__.build())"
+    args = list(ARGS)
+    args[args.index("-maxItr") + 1] = "2"
+    # K - omega0^2 M has no mass term to regularise it: once the design has near-void elements its solves
+    # are as ill-conditioned as a static one and stall above 1e-12, so these run at 1e-10
+    args[args.index("-ksp_rtol") + 1] = "1e-10"
+    hist = tmp_path / "h.jsonl"
+    cmd = [DRIVER, *args, "-precision", "0", "-reduction", "1", "-basis", "5", "-omega0", "0.01", "-history", str(hist)]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    its = [d for d in (json.loads(s) for s in hist.read_text().splitlines() if s.strip()) if "itr" in d]
+    assert [d["itr"] for d in its] == [1, 2]
+    _, pg = problem(nelx=32, nely=16, nelz=16, nsteps=100, dt=0.01, pulse_steps=20, nlvls=3)
+    c = capi.Context(pg)
+    _, xphys = c.filter_project(np.full(32 * 16 * 16, 0.12), projection=True, beta=0.1, eta=0.15)
+    c.close()
+    po = O.cantilever_params(32, 16, 16, nsteps=100, dt=0.01, pulse_steps=20)
+    po.rtol = 1e-12
+    ref = O.reduced_model(po, xphys, 5, 0.01)
+    assert abs(its[0]["fx"] - ref["fx"]) <= 1e-7 * abs(ref["fx"])
+    assert (tmp_path / "outputPhysics0_00000.dat").stat().st_size > 0

@@ -14,6 +14,7 @@
 //     coefficients; the outer iteration, its residual and dot products stay
 //     FP64, so the converged answer is an FP64 answer.
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 
@@ -627,10 +628,12 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
     stats.max_relres = std::max(stats.max_relres, rel);
     res.iters = it;
     res.relres = rel;
-    if (!conv)
-        throw Error{TT_ERR_NOT_CONVERGED, "flexible CG did not reach rtol " + std::to_string(rtol) + " in " +
-                                              std::to_string(prm.maxit) + " iterations (relres " +
-                                              std::to_string(rel) + ")"};
+    if (!conv) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "flexible CG did not reach rtol %.3g in %d iterations (relres %.3g)", rtol,
+                      prm.maxit, rel);
+        throw Error{TT_ERR_NOT_CONVERGED, msg};
+    }
     return res;
 }
 
