@@ -15,7 +15,8 @@
  *   - MMA supports one constraint (m = 1);
  *   - restart files (Restart0x.dat, Restart0x_itr_f0.dat, RestartSol0x.dat) are written and read in
  *     PETSc's binary Vec format with the reference's options; the cp%04d.dat checkpoint files are
- *     not written (the binomial checkpoints stay in device memory), and -solutionAsU0 is rejected;
+ *     not written (the binomial checkpoints stay in device memory); -onlyLoadDesign -solutionAsU0 reads
+ *     the initial displacement from -restartFileVecSol as the reference does;
  *   - the reduced-order path (reduction = TRUE, SOMMG) runs with the reduced systems solved directly,
  *     as in the reference; it reads and writes no solution restart file (RestartSol0x.dat);
  *   - numerical non-convergence and non-finite values are reported as errors (< 0).

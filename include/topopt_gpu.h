@@ -129,6 +129,12 @@ int tt_complete_initial_state(tt_ctx* ctx);
    whole Newmark forward sweep, keeping binomial checkpoints (or every state). */
 int tt_forward(tt_ctx* ctx);
 
+/* Initial displacement u0 (nd values; NULL: back to u0 = 0) of the next tt_complete_initial_state:
+   -solutionAsU0 with -onlyLoadDesign, where LoadSolutionAsU0 reads U0 from -restartFileVecSol and the
+   constructor masks it to N.U0 (LinearElasticity.cc:522-523, 584, 2621-2676); v0 stays 0.  The stored
+   value is N.u0.  TT_ERR_STATE with the reduced-order model. */
+int tt_set_initial_displacement(tt_ctx* ctx, const double* u0, int on_device);
+
 /* State at step t (0..nsteps) of the last tt_forward (GetStateFieldU/DU/DDU, LinearElasticity.h:68-78);
    3*nn doubles each (may be NULL).  A state that the binomial checkpoints did not keep is recomputed
    from the nearest stored one; asking for t = 1, 2, ... in turn costs one Newmark step each.  With the

@@ -94,6 +94,7 @@ def lib():
             "or_pcg": (ctypes.c_int, [P, dp, ctypes.c_double, ctypes.c_double, dp, dp, dp,
                                       ctypes.c_double, ctypes.c_int, dp]),
             "or_forward": (ctypes.c_int, [P, dp, dp]),
+            "or_forward_u0": (ctypes.c_int, [P, dp, dp, dp]),
             "or_objective_gradient": (ctypes.c_int, [P, dp, dp, dp, dp]),
             "or_objective": (ctypes.c_double, [P, dp, dp]),
             "or_pde_filter": (ctypes.c_int, [P, ctypes.c_double, dp, dp, ctypes.c_double]),
@@ -228,11 +229,15 @@ def pcg(p, xphys, cK, cM, b, rtol=1e-12, maxit=100000, x0=None):
     return x, it, rel.value
 
 
-def forward(p, xphys):
+def forward(p, xphys, u0=None):
+    """States (nsteps+1, 3, nd); u0: initial displacement (masked to the free dofs), default 0."""
     xphys = _f64(xphys)
     nd = 3 * nn(p)
     states = np.zeros((p.nsteps + 1) * 3 * nd)
-    st = lib().or_forward(ctypes.byref(p), _p(xphys), _p(states))
+    if u0 is None:
+        st = lib().or_forward(ctypes.byref(p), _p(xphys), _p(states))
+    else:
+        st = lib().or_forward_u0(ctypes.byref(p), _p(xphys), _p(_f64(u0)), _p(states))
     if st != 0:
         raise RuntimeError("oracle forward: CG did not converge")
     return states.reshape(p.nsteps + 1, 3, nd)

@@ -514,7 +514,8 @@ static void add_B(const ctx_t* c, double sK, double sM, const double* N, const d
     for (long i = 0; i < c->nd; i++) acc[i] += N[i] * t2[i];
 }
 
-int or_forward(const or_params* p, const double* xphys, double* states) {
+/* u0_in: initial displacement (NULL = 0), masked to N.u0 as LoadSolutionAsU0 + :584 leave it */
+int or_forward_u0(const or_params* p, const double* xphys, const double* u0_in, double* states) {
     ctx_t c;
     ctx_init(&c, p, xphys);
     long nd = c.nd;
@@ -532,6 +533,8 @@ int or_forward(const or_params* p, const double* xphys, double* states) {
     double* v0 = states + nd;
     double* a0 = states + 2 * nd;
     memset(states, 0, sizeof(double) * 3 * nd); /* U0 = DU0 = 0 (:521-525) */
+    if (u0_in)
+        for (long i = 0; i < nd; i++) u0[i] = N[i] * u0_in[i]; /* -solutionAsU0 (:523, 584, 2621-2676) */
     /* R = N.(f(0) - N.C.N v0 - N.K.N u0); M~ a0 = R  (:1050-1078) */
     double s0 = or_load_intensity(p, 0);
     for (long i = 0; i < nd; i++) R[i] = F[i] * s0;
@@ -577,6 +580,10 @@ int or_forward(const or_params* p, const double* xphys, double* states) {
     free(N); free(F); free(R); free(t1); free(t2);
     ctx_free(&c);
     return status;
+}
+
+int or_forward(const or_params* p, const double* xphys, double* states) {
+    return or_forward_u0(p, xphys, NULL, states);
 }
 
 /* ------------------------------------------------------------------ */

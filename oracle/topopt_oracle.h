@@ -80,6 +80,8 @@ int or_pcg(const or_params* p, const double* xphys, double cK, double cM, const 
 
 /* forward Newmark: states[(t*3 + {0,1,2})*nd + i] for t = 0..nsteps (u, v, a) */
 int or_forward(const or_params* p, const double* xphys, double* states);
+/* the same from a nonzero initial displacement N.u0 (-solutionAsU0, LinearElasticity.cc:523, 584, 2621-2676) */
+int or_forward_u0(const or_params* p, const double* xphys, const double* u0, double* states);
 /* objective and sensitivity (ComputeObjectiveSensitivities, LinearElasticity.cc:1625-1908) given states */
 int or_objective_gradient(const or_params* p, const double* xphys, const double* states,
                           double* fx, double* dfdx);

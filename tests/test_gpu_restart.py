@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from tests.gpu_helpers import problem, random_density, rel
 from tests.test_gpu_design_loop import ARGS, DRIVER
 from topopt_in_petsc_transient_amd import capi
@@ -115,3 +116,71 @@ def test_driver_restart_continues_the_design_loop(tmp_path):
         assert abs(d["fx_scaled"] - e["fx_scaled"]) <= 1e-9 * abs(e["fx_scaled"]), d["itr"]
         assert abs(d["ch"] - e["ch"]) <= 1e-7, d["itr"]
     assert rel(x_rest, x_full) < 1e-8
+
+
+def test_initial_displacement_matches_the_oracle():
+    """A nonzero initial displacement (-solutionAsU0, LinearElasticity.cc:523, 584, 2621-2676): states,
+    objective and sensitivities against the oracle, revolve against store-all, and the reset to u0 = 0."""
+    po, pg = problem(nsteps=20, slots=4)
+    n = pg.nelx * pg.nely * pg.nelz
+    x = random_density(n, seed=5)
+    nd = 3 * O.nn(po)
+    u0 = np.random.default_rng(6).standard_normal(nd) * 1e-2
+    ref = O.forward(po, x, u0=u0)
+    fx_ref, df_ref = O.objective_gradient(po, x, ref)
+    out = {}
+    for slots in (4, 0):
+        pg.checkpoint_slots = slots
+        c = _ctx(pg, x)
+        c.set_initial_displacement(u0)
+        c.complete_initial_state()
+        c.forward()
+        for t in (0, 5, 20):
+            u, v, a = c.get_state(t)
+            assert rel(u, ref[t, 0]) < 1e-8 and rel(v, ref[t, 1]) < 1e-8 and rel(a, ref[t, 2]) < 1e-8, (slots, t)
+        fx, df = c.objective_gradient()
+        assert abs(fx - fx_ref) <= 1e-9 * abs(fx_ref) and rel(df, df_ref) < 1e-8, slots
+        out[slots] = (fx, df)
+        if slots == 0:
+            c.set_initial_displacement(None)
+            c.complete_initial_state()
+            c.forward()
+            assert rel(c.get_state(20)[0], O.forward(po, x)[20, 0]) < 1e-8
+        c.close()
+    assert out[4][0] == out[0][0] and np.array_equal(out[4][1], out[0][1])
+
+
+def test_driver_reads_the_initial_displacement(tmp_path):
+    """-onlyLoadDesign -solutionAsU0 -restartFileVecSol f: the driver's first objective is the library's
+    with that initial displacement on the first design."""
+    assert os.path.exists(DRIVER), "build the driver first (__graft_entry__.build())"
+    nd = 3 * 33 * 17 * 17
+    u0 = np.random.default_rng(9).standard_normal(nd) * 1e-3
+    f = tmp_path / "u0.dat"
+    capi.petsc_write_vecs(f, [u0])
+    args = list(ARGS)
+    args[args.index("-maxItr") + 1] = "1"
+    hist = tmp_path / "h.jsonl"
+    cmd = [DRIVER, *args, "-precision", "0", "-history", str(hist), "-onlyLoadDesign", "1", "-solutionAsU0", "1",
+           "-restartFileVecSol", str(f), "-outputPhysics0", "0", "-outputPhysics1", "0"]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    its = [d for d in (json.loads(s) for s in hist.read_text().splitlines() if s.strip()) if "itr" in d]
+    _, pg = problem(nelx=32, nely=16, nelz=16, nsteps=100, dt=0.01, pulse_steps=20, nlvls=3)
+    c = capi.Context(pg)
+    _, xphys = c.filter_project(np.full(32 * 16 * 16, 0.12), projection=True, beta=0.1, eta=0.15)
+    c.set_density(xphys)
+    c.complete_initial_state()
+    c.forward()
+    fx0, _ = c.objective_gradient()
+    c.set_initial_displacement(u0)
+    c.complete_initial_state()
+    c.forward()
+    fx, _ = c.objective_gradient()
+    c.close()
+    assert abs(its[0]["fx"] - fx) <= 1e-9 * abs(fx)
+    # the initial displacement changes the response (by 3.6e-7 relative for this noise), far above the
+    # solver tolerance's effect on fx
+    assert abs(fx - fx0) > 1e-8 * abs(fx0)

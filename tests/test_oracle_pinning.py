@@ -239,6 +239,40 @@ def test_newmark_step_satisfies_the_equation_of_motion():
         assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(K @ u)
 
 
+def test_nonzero_initial_displacement_is_masked_and_starts_the_motion():
+    """-solutionAsU0: u0 = N.u0_in, v0 = 0, M a0 = N.(f(0) - K u0) on free dofs, and every later step
+    satisfies the equation of motion."""
+    p, x = _small(nsteps=6, rtol=1e-14)
+    nd = 3 * O.nn(p)
+    u0 = np.random.default_rng(7).standard_normal(nd) * 1e-2
+    st = O.forward(p, x, u0=u0)
+    N = O.bc_mask(p)
+    assert np.array_equal(st[0, 0], N * u0) and not st[0, 1].any()
+    K = O.assemble(p, x, 1.0, 0.0)
+    M = O.assemble(p, x, 0.0, 1.0)
+    C = p.dalpha * M + p.dbeta * K
+    F = O.load_location(p)
+    for t in range(0, p.nsteps + 1):
+        u, v, a = st[t]
+        r = (M @ a + C @ v + K @ u - F * O.load_intensity(p, t)) * N
+        assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(K @ u), t
+    assert np.linalg.norm(st[3, 0] - O.forward(p, x)[3, 0]) > 1e-3 * np.linalg.norm(st[3, 0])
+
+
+def test_adjoint_gradient_from_a_nonzero_initial_displacement():
+    p, x = _small(nsteps=6, rtol=1e-14)
+    u0 = np.random.default_rng(8).standard_normal(3 * O.nn(p)) * 1e-2
+    fx, df = O.objective_gradient(p, x, O.forward(p, x, u0=u0))
+    rng = np.random.default_rng(12)
+    h = 1e-6
+    for _ in range(2):
+        d = rng.standard_normal(len(x))
+        d /= np.linalg.norm(d)
+        fp = O.objective(p, x + h * d, O.forward(p, x + h * d, u0=u0))
+        fm = O.objective(p, x - h * d, O.forward(p, x - h * d, u0=u0))
+        assert df @ d == pytest.approx((fp - fm) / (2 * h), rel=2e-6, abs=1e-9 * abs(fx))
+
+
 # ---------------------------------------------------------------- A13-A18 adjoint sensitivities
 @pytest.mark.parametrize("objective", [3, 0, 1, 2, 4, 5, 6])
 def test_adjoint_gradient_matches_central_differences(objective):

@@ -27,7 +27,7 @@ EXPORTED = [
     "tt_mma_kkt_residual", "tt_mma_get_asymptotes", "tt_get_stats", "tt_reset_stats", "tt_synchronize",
     "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
     "tt_mma_get_state", "tt_mma_set_state", "tt_get_restart_vectors", "tt_set_restart_vectors",
-    "tt_petsc_write_vecs", "tt_petsc_read_vecs", "tt_get_reduced_model",
+    "tt_petsc_write_vecs", "tt_petsc_read_vecs", "tt_get_reduced_model", "tt_set_initial_displacement",
 ]
 
 
@@ -104,6 +104,7 @@ def lib():
             "tt_petsc_write_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
             "tt_petsc_read_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
             "tt_get_reduced_model": [vp, vp, ip, vp, vp, vp, vp, vp, vp],
+            "tt_set_initial_displacement": [vp, vp, ip],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -261,6 +262,16 @@ class Context:
         out = [np.zeros(self.nd) for _ in range(3)]
         self._chk(lib().tt_get_restart_vectors(self.h, *[o.ctypes.data for o in out]))
         return tuple(out)
+
+    def set_initial_displacement(self, u0=None):
+        """u0 (nd values, masked to the free dofs) for the next complete_initial_state; None resets to 0."""
+        if u0 is None:
+            self._chk(lib().tt_set_initial_displacement(self.h, None, 0))
+            return
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        if u0.size != self.nd:
+            raise ValueError("u0 has nd values")
+        self._chk(lib().tt_set_initial_displacement(self.h, u0.ctypes.data, 0))
 
     def reduced_model(self, with_states=False):
         """The SOMMG reduced model (reduction = 1): dict of Q (basis x nd), Ut, Kr, Mr, Cr (basis x basis),

@@ -634,8 +634,22 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     PetscOptionsGetBool(NULL, NULL, "-onlyLoadDesign", &onlyDesign, &flg);
     PetscOptionsGetBool(NULL, NULL, "-solutionAsU0", &solutionAsU0, &flg);
     if (onlyDesign && solutionAsU0) {
-        std::fprintf(stderr, "[topopt] -solutionAsU0 (a nonzero initial displacement) is not part of this build\n");
-        std::exit(1);
+        // LoadSolutionAsU0 (LinearElasticity.cc:2621-2676): the first vector of -restartFileVecSol becomes
+        // the initial displacement, masked to the free dofs (:584)
+        const std::string fileU0 = option_string("-restartFileVecSol", "");
+        PetscPrintf(PETSC_COMM_WORLD, "# Loaded solution from %s as initial displacements U0 (-restartFileVecSol):  \n",
+                    fileU0.c_str());
+        if (!file_exists(fileU0)) {
+            PetscPrintf(PETSC_COMM_WORLD, "# File: %s NOT FOUND \n", fileU0.c_str());
+        } else {
+            const long nd = 3 * tt_num_nodes(ctx);
+            std::vector<double> u0(nd);
+            double* v[1] = {u0.data()};
+            const long lens[1] = {nd};
+            if (report(nullptr, tt_petsc_read_vecs(fileU0.c_str(), 1, v, lens), "tt_petsc_read_vecs") != TT_OK ||
+                report(ctx, tt_set_initial_displacement(ctx, u0.data(), 0), "tt_set_initial_displacement") != TT_OK)
+                std::exit(1);
+        }
     }
     const std::string wd = option_string("-workdir", "./");
     filename00 = wd + "/RestartSol00.dat";

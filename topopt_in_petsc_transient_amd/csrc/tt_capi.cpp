@@ -782,6 +782,26 @@ int tt_mma_set_state(tt_mma* m, int k, const double* xo1, const double* xo2, con
     TT_API_END
 }
 
+int tt_set_initial_displacement(tt_ctx* ctx, const double* u0, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(!C.reduced, TT_ERR_STATE, "no initial displacement with the reduced-order model");
+    C.initial_done = false;
+    C.forward_done = false;
+    const long nd = C.g.nd;
+    C.u0_set = false;
+    if (u0) {
+        if (!C.u0in.p) C.u0in.alloc(nd);
+        TT_HIP(hipMemcpyAsync(C.u0in.p, u0, nd * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              C.stream));
+        // N.u0 (:584): the boundary-condition form with zero coefficients returns (1 - N).u0
+        tt::launch_apply_bc(C.op_fine64, 0.0, 0.0, C.u0in.p, C.rhs.p, true, C.stream);
+        tt::launch_lincomb(1.0, C.u0in.p, -1.0, C.rhs.p, C.u0in.p, nd, C.stream);
+        C.u0_set = true;
+        C.sync();
+    }
+    TT_API_END
+}
+
 int tt_get_reduced_model(tt_ctx* ctx, double* Q, int on_device, double* Ut, double* Kr, double* Mr, double* Cr,
                          double* Pr, double* rstates) {
     TT_API_BEGIN(ctx)

@@ -104,6 +104,8 @@ struct Context {
     bool density_set = false;
     bool initial_done = false;
     bool a0_guess = false;  // slot 0 holds a starting guess for a0 (tt_set_restart_vectors), used once
+    DBuf<double> u0in;      // initial displacement N.u0 (tt_set_initial_displacement), used when u0_set
+    bool u0_set = false;
     int scratch_step = -1;  // st_cur holds this state, recomputed by tt_get_state (-1: nothing)
     const double* state_at(int t);  // state t: stored, or recomputed from the nearest stored state
     bool forward_done = false;

@@ -47,11 +47,13 @@ void Context::complete_initial_state() {
     const long nd = g.nd;
     mg.setup_operator(cK_A, cM_A);
     launch_diag_inv(op_fine64, 0.0, 1.0, dinvM.p, true, stream);
-    // state 0: u0 = v0 = 0 (LinearElasticity.cc:521-525); M~ a0 = N.(f(0) - C v0 - K u0) (:1050-1078)
+    // state 0: u0 = v0 = 0 (LinearElasticity.cc:521-525), or the u0 of tt_set_initial_displacement;
+    // M~ a0 = N.(f(0) - C v0 - K u0) (:1050-1078)
     double* s0 = slots[0].p;
     // u0 = v0 = 0 and no acceleration history; the solve starts from a0 = 0, or once from the a0 that
     // tt_set_restart_vectors supplied (the reference's DDU0 read from RestartSol0x.dat, :2716-2720)
     launch_fill(s0, 0.0, 2 * nd, true, stream);
+    if (u0_set) launch_copy(u0in.p, s0, nd, true, true, stream);  // -solutionAsU0 (:523, 584, 2621-2676)
     launch_fill(s0 + 3 * nd, 0.0, (kStateVecs - 3) * nd, true, stream);
     if (!a0_guess) launch_fill(s0 + 2 * nd, 0.0, nd, true, stream);
     a0_guess = false;
