@@ -7,6 +7,8 @@ import subprocess
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from oracle import oracle as O
 from tests.gpu_helpers import problem, random_density, rel
@@ -153,3 +155,39 @@ def test_drop_in_driver_runs_the_reduced_model(tmp_path):
     ref = O.reduced_model(po, xphys, 5, 0.01)
     assert abs(its[0]["fx"] - ref["fx"]) <= 1e-7 * abs(ref["fx"])
     assert (tmp_path / "outputPhysics0_00000.dat").stat().st_size > 0
+
+
+def _direct_basis(po, x, nb, om):
+    """The basis sequence with scipy's direct solver on the assembled matrices (any sign of K - om^2 M)."""
+    K, M, N = O.assemble(po, x, 1.0, 0.0), O.assemble(po, x, 0.0, 1.0), O.bc_mask(po)
+    Nd = sp.diags(N)
+    P0 = (Nd @ (K - om * om * M) @ Nd + sp.diags(1.0 - N)).tocsc()
+    f = O.load_location(po) * N
+    Q = []
+    for j in range(nb):
+        if j > 0:
+            f = N * (2.0 * om * (M @ Q[j - 1]))
+        v = spla.spsolve(P0, f)
+        for q in Q:
+            v = v - (q @ v) * q
+        Q.append(v / np.linalg.norm(v))
+    return np.array(Q)
+
+
+def test_basis_above_the_first_eigenfrequency():
+    """omega0 = 0.2 lies above the first two eigenfrequencies of this design (0.138 and 0.145 rad/s), so
+    K - omega0^2 M is indefinite: the flexible GMRES basis solves (shifted-operator multigrid) still give
+    the basis, the reduced model and its gradient of the direct solver."""
+    po, pg, x = _setup(nsteps=12)
+    pg.omega0 = 0.2
+    print("reduced model, indefinite basis operator", flush=True)
+    c, m, fx, df = _run(pg, x)
+    c.close()
+    Q = _direct_basis(po, x, 5, 0.2)
+    assert rel(m["Q"], Q) < 1e-8
+    Kr, Mr, Cr, Pr = O.sommg_reduce(po, x, Q)
+    rs = O.reduced_forward(po, Kr, Mr, Cr, Pr)
+    fx_ref, df_ref = O.reduced_objective_gradient(po, x, Q, Kr, Mr, Cr, rs)
+    assert rel(m["Kr"], Kr) < 1e-8 and rel(m["Pr"], Pr) < 1e-8
+    assert abs(fx - fx_ref) <= 1e-8 * abs(fx_ref)
+    assert rel(df, df_ref) < 1e-7

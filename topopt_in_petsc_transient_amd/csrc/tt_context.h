@@ -161,6 +161,8 @@ struct Context {
     void expand(const double* coef, double* out);  // out = sum_k coef[k] Q_k
     void expand_state(int t, double* out);    // u, v, a of reduced state t on the full mesh
     void project3(const double* h0, const double* h1, const double* h2, double* out);  // out[3k+i] = Q_k.h_i
+    DBuf<double> gmV, gmZ;  // flexible-GMRES Krylov and preconditioned vectors of the basis solves
+    SolveResult fgmres(double cK, double cM, const double* b, double* x);
     void reduced_adjoint(const std::function<void(const double*)>& grad_state,
                          const std::function<void(int, const double*, const double*, const double*, int)>& sens);
 

@@ -6,7 +6,9 @@
 // moment-matching correction to that right-hand side as fvec += (-1.0)*(-1,0)*massVec (:176); the
 // comma expression (-1,0) is 0, so the correction never changes fvec and is not formed here.  The
 // basis solves are the multigrid-preconditioned flexible CG of the full model with the operator
-// scalars (1, -omega0^2) (the reference hands its multigrid KSP the matrix P0, :63-66).
+// scalars (1, -omega0^2), like the reference's FGMRES KSP with multigrid (:63-66; LinearElasticity.cc:2767);
+// the multigrid here is that of the shifted operator K + omega0^2 M, which stays positive definite when
+// omega0 lies above the design's first eigenfrequency and P0 is indefinite.
 //
 // Reduced model: Kr = Q^T K Q, Mr = Q^T M Q, Cr = Q^T C Q, Pr = Q^T N.F (:236-262), then the same
 // Newmark and adjoint recurrences as the full model on these nb x nb matrices, solved by LU
@@ -23,6 +25,7 @@
 // recurrences themselves are a few nb x nb products per step on the host.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -104,12 +107,103 @@ void Context::project3(const double* h0, const double* h1, const double* h2, dou
     }
 }
 
+// Flexible GMRES(kRestart) with right preconditioning z_j = B v_j, B the multigrid V-cycle as set up
+// (modified Gram-Schmidt Arnoldi, Givens rotations; the true residual is formed at every restart and
+// decides convergence: ||b - A x|| <= rtol ||b||).
+static constexpr int kRestart = 30;
+SolveResult Context::fgmres(double cK, double cM, const double* b, double* x) {
+    const long nd = g.nd;
+    const int m = kRestart;
+    if (!gmV.p) {
+        gmV.alloc((size_t)(m + 1) * nd);
+        gmZ.alloc((size_t)m * nd);
+    }
+    auto V = [&](int i) { return gmV.p + (size_t)i * nd; };
+    auto Z = [&](int i) { return gmZ.p + (size_t)i * nd; };
+    auto dot = [&](const double* p, const double* q) {
+        launch_dot(p, q, nd, red, PB_MISC, stream);
+        return reduce1(PB_MISC);
+    };
+    const double rtol = prm.rtol;
+    const double bn = std::sqrt(dot(b, b));
+    SolveResult res;
+    stats.solves++;
+    if (bn == 0.0) {
+        launch_fill(x, 0.0, nd, true, stream);
+        return res;
+    }
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+    int it = 0;
+    double rel = 1.0;
+    while (true) {
+        launch_residual(op_fine64, cK, cM, b, x, V(0), true, stream);
+        const double beta = std::sqrt(dot(V(0), V(0)));
+        rel = beta / bn;
+        if (rel <= rtol) break;
+        if (it >= prm.maxit || !std::isfinite(rel)) break;
+        launch_scale(V(0), nd, 1.0 / beta, stream);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int k = 0;
+        for (; k < m && it < prm.maxit; k++) {
+            mg.apply(V(k), Z(k));
+            launch_apply_bc(op_fine64, cK, cM, Z(k), V(k + 1), true, stream);
+            for (int i = 0; i <= k; i++) {
+                const double h = dot(V(k + 1), V(i));
+                H[(size_t)i * m + k] = h;
+                launch_lincomb(1.0, V(k + 1), -h, V(i), V(k + 1), nd, stream);
+            }
+            const double hn = std::sqrt(dot(V(k + 1), V(k + 1)));
+            H[(size_t)(k + 1) * m + k] = hn;
+            if (hn > 0.0) launch_scale(V(k + 1), nd, 1.0 / hn, stream);
+            for (int i = 0; i < k; i++) {  // earlier rotations on the new column
+                const double a0 = H[(size_t)i * m + k], a1 = H[(size_t)(i + 1) * m + k];
+                H[(size_t)i * m + k] = cs[i] * a0 + sn[i] * a1;
+                H[(size_t)(i + 1) * m + k] = -sn[i] * a0 + cs[i] * a1;
+            }
+            const double a0 = H[(size_t)k * m + k], a1 = H[(size_t)(k + 1) * m + k];
+            const double r = std::hypot(a0, a1);
+            cs[k] = r > 0.0 ? a0 / r : 1.0;
+            sn[k] = r > 0.0 ? a1 / r : 0.0;
+            H[(size_t)k * m + k] = r;
+            H[(size_t)(k + 1) * m + k] = 0.0;
+            gv[k + 1] = -sn[k] * gv[k];
+            gv[k] = cs[k] * gv[k];
+            it++;
+            if (std::fabs(gv[k + 1]) <= rtol * bn || hn == 0.0) {
+                k++;
+                break;
+            }
+        }
+        // y = R^-1 g (upper triangular k x k), x += Z y
+        for (int i = k - 1; i >= 0; i--) {
+            double s = gv[i];
+            for (int j = i + 1; j < k; j++) s -= H[(size_t)i * m + j] * y[j];
+            y[i] = s / H[(size_t)i * m + i];
+        }
+        for (int i = 0; i < k; i++) launch_lincomb(1.0, x, y[i], Z(i), x, nd, stream);
+    }
+    stats.krylov_iters += it;
+    stats.last_iters = it;
+    stats.last_relres = rel;
+    stats.max_relres = std::max(stats.max_relres, rel);
+    res.iters = it;
+    res.relres = rel;
+    if (!(rel <= rtol)) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "flexible GMRES did not reach rtol %.3g in %d iterations (relres %.3g)", rtol,
+                      prm.maxit, rel);
+        throw Error{TT_ERR_NOT_CONVERGED, msg};
+    }
+    return res;
+}
+
 void Context::reduced_initial_state() {
     const long nd = g.nd;
     const int nb = nbasis;
     const double om = prm.omega0;
     const double cK0 = 1.0, cM0 = -om * om;
-    mg.setup_operator(cK0, cM0);
+    mg.setup_operator(1.0, om * om);  // preconditioner: the positive-definite shifted operator
     // f0 = N.F: the BC form with zero coefficients returns (1 - N).F
     double* f0 = rhs.p;
     double* f = Kl.p;
@@ -126,7 +220,7 @@ void Context::reduced_initial_state() {
             rhs_j = f;
         }
         launch_fill(qb, 0.0, nd, true, stream);
-        fcg(cK0, cM0, true, kSolveForward, rhs_j, qb);
+        fgmres(cK0, cM0, rhs_j, qb);
         for (int i = 0; i < j; i++) {
             const double* qi = Qb.p + (size_t)i * nd;
             launch_dot(qi, qb, nd, red, PB_MISC, stream);
