@@ -245,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-reduced", action="store_true", help="skip the SOMMG reduced-order measurement")
+    ap.add_argument("--reduced-only", action="store_true",
+                    help="only the SOMMG reduced-order gradient, mixed precision (for profiling)")
     ap.add_argument("--ops-only", action="store_true", help="only the operator microbenchmark (for profiling)")
     ap.add_argument("--nlvls", type=int, default=4, help="multigrid levels (4: coarsest grid 16x8x8 elements)")
     ap.add_argument("--cheb-mixed", type=int, default=None, help="Chebyshev degree of the mixed-precision multigrid")
@@ -269,6 +271,11 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     log("density field")
     x = density(capi)
+    if args.reduced_only:
+        rr, fxr, _ = reduced_run(capi, x, capi.TT_PREC_MIXED, args.steps, 5, 0.01)
+        rr["fx"] = fxr
+        print(json.dumps({"reduced_model": {"mixed": rr}}))
+        return
     log("operator microbenchmark")
     ops = op_microbench(capi, torch, args.nrep)
     if args.ops_only:
