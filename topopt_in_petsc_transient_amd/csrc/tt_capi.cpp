@@ -176,6 +176,8 @@ Context::Context(const tt_params& p) : prm(p) {
         nslots = 1;
         nbasis = p.basis;
         Qb.alloc((size_t)nbasis * nd);
+        projpart.alloc((size_t)nbasis * 3 * 1024);
+        TT_HIP(hipHostMalloc((void**)&projhost, sizeof(double) * nbasis * 3 * 1024, hipHostMallocDefault));
     }
     store_all = (nslots == T + 1);
     size_t freeb = 0, totalb = 0;
@@ -213,6 +215,7 @@ void Context::set_filter_radius(double rmin) {
 
 Context::~Context() {
     clear_graphs();
+    if (projhost) (void)hipHostFree(projhost);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

@@ -155,6 +155,8 @@ struct Context {
     bool reduced = false;
     int nbasis = 0;
     DBuf<double> Qb;
+    DBuf<double> projpart;          // projection partials: per basis vector 1024 blocks x 3
+    double* projhost = nullptr;     // pinned host copy of projpart
     std::vector<double> Ut, Kr, Mr, Cr, Pr, rst;
     void reduced_initial_state();             // basis, reduced matrices, a0 (CompleteInitialState)
     void reduced_forward();                   // reduced Newmark sweep (SolveStateBlock)

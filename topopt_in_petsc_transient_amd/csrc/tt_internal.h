@@ -237,6 +237,9 @@ void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot
 // q.a, q.b, q.c as three partial columns of one region
 void launch_dot3(const double* q, const double* a, const double* b, const double* c, long n, Reducer& red, int slot,
                  hipStream_t s);
+// the same into part[block*3 + i]; returns the number of blocks (at most 1024)
+int launch_dot3_to(const double* q, const double* a, const double* b, const double* c, long n, double* part,
+                   hipStream_t s);
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s);
 // Flexible-CG (re)start from the b.b and r.r partial regions: device tolerance (rtol ||b||)^2, the

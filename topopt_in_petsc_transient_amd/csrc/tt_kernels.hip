@@ -1313,6 +1313,14 @@ void launch_dot3(const double* q, const double* a, const double* b, const double
     TT_HIP(hipGetLastError());
 }
 
+int launch_dot3_to(const double* q, const double* a, const double* b, const double* c, long n, double* part,
+                   hipStream_t s) {
+    const int nb = rb_blocks(n);
+    hipLaunchKernelGGL(k_dot3, dim3(nb), dim3(BS), 0, s, q, a, b, c, n, part);
+    TT_HIP(hipGetLastError());
+    return nb;
+}
+
 // gamma = z.r, delta = z.r_prev (flexible CG, Polak-Ribiere form)
 __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ rp,
                            long n, double* part, int first) {

@@ -101,9 +101,19 @@ void Context::expand_state(int t, double* out) {
 }
 
 void Context::project3(const double* h0, const double* h1, const double* h2, double* out) {
+    // every basis vector's block partials into its own slice, one copy and one synchronisation, then
+    // the sums in block order
+    int nblk = 0;
+    for (int k = 0; k < nbasis; k++)
+        nblk = launch_dot3_to(Qb.p + (size_t)k * g.nd, h0, h1, h2, g.nd, projpart.p + (size_t)k * 3 * 1024, stream);
+    TT_HIP(hipMemcpyAsync(projhost, projpart.p, sizeof(double) * nbasis * 3 * 1024, hipMemcpyDeviceToHost, stream));
+    sync();
     for (int k = 0; k < nbasis; k++) {
-        launch_dot3(Qb.p + (size_t)k * g.nd, h0, h1, h2, g.nd, red, PB_MISC, stream);
-        red.host_sum(PB_MISC, stream, out + 3 * k);
+        const double* pk = projhost + (size_t)k * 3 * 1024;
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int b = 0; b < nblk; b++)
+            for (int i = 0; i < 3; i++) s[i] += pk[(size_t)b * 3 + i];
+        for (int i = 0; i < 3; i++) out[3 * k + i] = s[i];
     }
 }
 
