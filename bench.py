@@ -259,8 +259,8 @@ def main():
     if args.cheb_fp64:
         CHEB[0] = args.cheb_fp64
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU domain decomposition is a design note only).", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU: --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is described, not built).", file=sys.stderr)
         sys.exit(2)
     import torch
     from topopt_in_petsc_transient_amd import capi

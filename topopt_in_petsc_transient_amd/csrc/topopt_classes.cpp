@@ -121,7 +121,7 @@ PetscErrorCode DMDestroy(DM* dm) {
     return 0;
 }
 
-// ------------------------------------------------------------------ output writer (not part of this build)
+// ------------------------------------------------------------------ output writer
 // MPIIO.cc:32-136, 409-586 (constructor, Allocate, writePoints, writeCells) for one rank and one domain:
 // all of it is the file's first part, written here in one go.  With -workdir the file goes to
 // "./" + workdir + "/" + filename, as MPIIO.cc:431-437 builds it; an absolute workdir (which that
@@ -608,7 +608,7 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     p.region = optimizationRegion;
     p.dalpha = dalpha_in;
     p.dbeta = dbeta_in;
-    // options of this build (no reference counterpart): precision, state slots, solver, load signal
+    // options with no reference counterpart: precision, state slots, solver, load signal
     PetscBool flg;
     PetscOptionsGetInt(NULL, NULL, "-precision", &p.precision, &flg);
     PetscOptionsGetInt(NULL, NULL, "-slots", &p.checkpoint_slots, &flg);
@@ -767,7 +767,7 @@ PetscErrorCode LinearElasticity::WriteRestartFiles() {
     if (!restart) return -1;
     if (reduced) {
         // the reference writes the reduced DDU0, LT, DDL0 here, starting guesses of direct solves that
-        // need none; this build writes no solution restart file with the reduced-order model
+        // need none, so no solution restart file is written with the reduced-order model
         PetscPrintf(PETSC_COMM_WORLD, "# Reduced-order model: no solution restart file\n");
         return 0;
     }
@@ -813,7 +813,7 @@ PetscErrorCode PDEFilt::Gradients(Vec OS, Vec FS) {  // self-adjoint (PDEFilter.
 // ------------------------------------------------------------------ Filter (Filter.cc)
 Filter::Filter(DM da_nodes, Vec, PetscInt filterT, PetscScalar Rin) : filterType(filterT), R(Rin) {
     if (filterT != 2) {
-        std::fprintf(stderr, "[topopt] only the PDE filter (-filter 2) is part of this build (got %d)\n", filterT);
+        std::fprintf(stderr, "[topopt] only the PDE filter (-filter 2) is supported (got %d)\n", filterT);
         std::exit(1);
     }
     pdef = new PDEFilt(da_nodes, Rin);
