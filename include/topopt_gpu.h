@@ -138,7 +138,8 @@ int tt_set_initial_displacement(tt_ctx* ctx, const double* u0, int on_device);
 /* State at step t (0..nsteps) of the last tt_forward (GetStateFieldU/DU/DDU, LinearElasticity.h:68-78);
    3*nn doubles each (may be NULL).  A state that the binomial checkpoints did not keep is recomputed
    from the nearest stored one; asking for t = 1, 2, ... in turn costs one Newmark step each.  With the
-   reduced-order model the state is the reduced one expanded on the basis. */
+   reduced-order model the state is the reduced one expanded on the basis.  State 0 is available as soon
+   as tt_complete_initial_state has run. */
 int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device);
 
 /* ComputeObjectiveSensitivities (LinearElasticity.cc:1625-1908): objective fx and dfdx

@@ -162,7 +162,7 @@ def test_driver_reads_the_initial_displacement(tmp_path):
     args[args.index("-maxItr") + 1] = "1"
     hist = tmp_path / "h.jsonl"
     cmd = [DRIVER, *args, "-precision", "0", "-history", str(hist), "-onlyLoadDesign", "1", "-solutionAsU0", "1",
-           "-restartFileVecSol", str(f), "-outputPhysics0", "0", "-outputPhysics1", "0"]
+           "-restartFileVecSol", str(f), "-outputPhysics1", "0"]
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=tmp_path)
     print(r.stdout[-2000:], r.stderr[-2000:])
@@ -181,6 +181,12 @@ def test_driver_reads_the_initial_displacement(tmp_path):
     fx, _ = c.objective_gradient()
     c.close()
     assert abs(its[0]["fx"] - fx) <= 1e-9 * abs(fx)
+    # the field output's step 0 is the masked initial displacement (float32)
+    from tests.test_gpu_output import read_output
+    phys0 = read_output(tmp_path / "outputPhysics0_00000.dat")
+    assert phys0["data"][0][0] == 0
+    N = O.bc_mask(O.cantilever_params(32, 16, 16))
+    assert np.array_equal(np.stack(phys0["data"][0][1], 1).ravel(), (N * u0).astype(np.float32))
     # the initial displacement changes the response (by 3.6e-7 relative for this noise), far above the
     # solver tolerance's effect on fx
     assert abs(fx - fx0) > 1e-8 * abs(fx0)

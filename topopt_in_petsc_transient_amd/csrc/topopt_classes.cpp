@@ -701,9 +701,10 @@ PetscErrorCode LinearElasticity::CompleteInitialState(Vec xPhys, PetscScalar Emi
     TTCALL(ctx, tt_set_density(ctx, xPhys->a.data(), 0));
     TTCALL(ctx, tt_complete_initial_state(ctx));
     forward_done = false;
-    if (write && output) {  // the initial state (:1086-1113): u0 = 0
+    if (write && output) {  // the initial state (:1086-1113): u0 (0 unless -solutionAsU0 gave one)
         Vec U0;
         VecCreateSeq(0, 3 * (PetscInt)tt_num_nodes(ctx), &U0);
+        TTCALL(ctx, tt_get_state(ctx, 0, U0->a.data(), nullptr, nullptr, 0));
         ierr = output->WriteVTK(da_nodal, U0, xPhys, 0);
         VecDestroy(&U0);
         if (ierr) return ierr;

@@ -213,7 +213,7 @@ struct Revolve {
 // st_cur from the nearest stored state below t, or from the state recomputed last when that is
 // nearer, so that asking for t = 1, 2, ..., T in turn costs one Newmark step each.
 const double* Context::state_at(int t) {
-    TT_CHECK(forward_done, TT_ERR_STATE, "tt_forward must be called first");
+    TT_CHECK(forward_done || (t == 0 && initial_done), TT_ERR_STATE, "tt_forward must be called first");
     TT_CHECK(t >= 0 && t <= prm.nsteps, TT_ERR_ARG, "time step out of range");
     if (reduced) {  // the reduced state expanded on the full mesh
         expand_state(t, st_cur.p);
