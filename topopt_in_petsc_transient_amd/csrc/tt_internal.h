@@ -234,12 +234,10 @@ void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, 
 
 // dots (FP64): results land in red.scal[slot..]
 void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot, hipStream_t s);
-// q.a, q.b, q.c as three partial columns of one region
-void launch_dot3(const double* q, const double* a, const double* b, const double* c, long n, Reducer& red, int slot,
-                 hipStream_t s);
-// the same into part[block*3 + i]; returns the number of blocks (at most 1024)
-int launch_dot3_to(const double* q, const double* a, const double* b, const double* c, long n, double* part,
-                   hipStream_t s);
+// Q_k.a, Q_k.b, Q_k.c for the nq <= 8 vectors Q_k = Q + k*n, k0 <= k < k0 + nq, in one pass over a, b, c:
+// block partials into part[(k*1024 + block)*3 + i]; returns the number of blocks (at most 1024)
+int launch_proj3(const double* Q, int k0, int nq, const double* a, const double* b, const double* c, long n,
+                 double* part, hipStream_t s);
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s);
 // Flexible-CG (re)start from the b.b and r.r partial regions: device tolerance (rtol ||b||)^2, the

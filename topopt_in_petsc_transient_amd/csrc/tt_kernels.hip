@@ -1293,30 +1293,39 @@ void launch_dot(const double* a, const double* b, long n, Reducer& red, int regi
     TT_HIP(hipGetLastError());
 }
 
-__global__ void k_dot3(const double* __restrict__ q, const double* __restrict__ a, const double* __restrict__ b,
-                       const double* __restrict__ c, long n, double* part) {
-    double acc[3] = {0.0, 0.0, 0.0};
+template <int NQ>
+__global__ void k_proj3(const double* __restrict__ Q, int k0, const double* __restrict__ a,
+                        const double* __restrict__ b, const double* __restrict__ c, long n, double* part) {
+    double acc[3 * NQ];
+#pragma unroll
+    for (int j = 0; j < 3 * NQ; j++) acc[j] = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double qi = q[i];
-        acc[0] = fma(qi, a[i], acc[0]);
-        acc[1] = fma(qi, b[i], acc[1]);
-        acc[2] = fma(qi, c[i], acc[2]);
+        const double ai = a[i], bi = b[i], ci = c[i];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const double qi = Q[(size_t)(k0 + q) * n + i];
+            acc[3 * q] = fma(qi, ai, acc[3 * q]);
+            acc[3 * q + 1] = fma(qi, bi, acc[3 * q + 1]);
+            acc[3 * q + 2] = fma(qi, ci, acc[3 * q + 2]);
+        }
     }
-    write_partials<3>(acc, part);
+    __shared__ double sh[(BS / 64) * 3 * NQ];
+    block_sum<3 * NQ>(acc, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            for (int j = 0; j < 3; j++) part[((size_t)(k0 + q) * RB + blockIdx.x) * 3 + j] = acc[3 * q + j];
 }
-void launch_dot3(const double* q, const double* a, const double* b, const double* c, long n, Reducer& red, int region,
-                 hipStream_t s) {
+int launch_proj3(const double* Q, int k0, int nq, const double* a, const double* b, const double* c, long n,
+                 double* part, hipStream_t s) {
     const int nb = rb_blocks(n);
-    hipLaunchKernelGGL(k_dot3, dim3(nb), dim3(BS), 0, s, q, a, b, c, n, red.part(region));
-    red.nblk[region] = nb;
-    red.nv[region] = 3;
-    TT_HIP(hipGetLastError());
-}
-
-int launch_dot3_to(const double* q, const double* a, const double* b, const double* c, long n, double* part,
-                   hipStream_t s) {
-    const int nb = rb_blocks(n);
-    hipLaunchKernelGGL(k_dot3, dim3(nb), dim3(BS), 0, s, q, a, b, c, n, part);
+    switch (nq) {
+#define TT_PROJ3(NQ) \
+    case NQ: hipLaunchKernelGGL(k_proj3<NQ>, dim3(nb), dim3(BS), 0, s, Q, k0, a, b, c, n, part); break;
+        TT_PROJ3(1) TT_PROJ3(2) TT_PROJ3(3) TT_PROJ3(4) TT_PROJ3(5) TT_PROJ3(6) TT_PROJ3(7) TT_PROJ3(8)
+#undef TT_PROJ3
+    default: throw Error{-3, "launch_proj3: 1..8 vectors per launch"};  // TT_ERR_INTERNAL
+    }
     TT_HIP(hipGetLastError());
     return nb;
 }

@@ -101,11 +101,11 @@ void Context::expand_state(int t, double* out) {
 }
 
 void Context::project3(const double* h0, const double* h1, const double* h2, double* out) {
-    // every basis vector's block partials into its own slice, one copy and one synchronisation, then
-    // the sums in block order
+    // one pass over h0, h1, h2 per 8 basis vectors; every vector's block partials into its own slice,
+    // one copy and one synchronisation, then the sums in block order
     int nblk = 0;
-    for (int k = 0; k < nbasis; k++)
-        nblk = launch_dot3_to(Qb.p + (size_t)k * g.nd, h0, h1, h2, g.nd, projpart.p + (size_t)k * 3 * 1024, stream);
+    for (int k0 = 0; k0 < nbasis; k0 += 8)
+        nblk = launch_proj3(Qb.p, k0, std::min(8, nbasis - k0), h0, h1, h2, g.nd, projpart.p, stream);
     TT_HIP(hipMemcpyAsync(projhost, projpart.p, sizeof(double) * nbasis * 3 * 1024, hipMemcpyDeviceToHost, stream));
     sync();
     for (int k = 0; k < nbasis; k++) {
