@@ -238,6 +238,14 @@ void launch_dot(const double* a, const double* b, long n, Reducer& red, int slot
 // block partials into part[(k*1024 + block)*3 + i]; returns the number of blocks (at most 1024)
 int launch_proj3(const double* Q, int k0, int nq, const double* a, const double* b, const double* c, long n,
                  double* part, hipStream_t s);
+// out[o] = sum_k c[o][k] Q_k for nout <= 3 outputs and nb <= 64 vectors Q_k = Q + k*n, one pass over Q
+struct Expand3 {
+    const double* Q;
+    double* out[3];
+    double c[3][64];
+    int nb, nout;
+};
+void launch_expand3(const Expand3& e, long n, hipStream_t s);
 void launch_fcg_dots(const double* z, const double* r, const double* rprev, long n, Reducer& red, bool first,
                      hipStream_t s);
 // Flexible-CG (re)start from the b.b and r.r partial regions: device tolerance (rtol ||b||)^2, the

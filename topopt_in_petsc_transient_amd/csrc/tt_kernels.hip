@@ -1293,6 +1293,26 @@ void launch_dot(const double* a, const double* b, long n, Reducer& red, int regi
     TT_HIP(hipGetLastError());
 }
 
+__global__ void k_expand3(Expand3 e, long n) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = 0; k < e.nb; k++) {
+        const double q = e.Q[(size_t)k * n + i];
+        s0 = fma(e.c[0][k], q, s0);
+        s1 = fma(e.c[1][k], q, s1);
+        s2 = fma(e.c[2][k], q, s2);
+    }
+    e.out[0][i] = s0;
+    if (e.nout > 1) e.out[1][i] = s1;
+    if (e.nout > 2) e.out[2][i] = s2;
+}
+void launch_expand3(const Expand3& e, long n, hipStream_t s) {
+    TT_CHECK(e.nb >= 1 && e.nb <= 64 && e.nout >= 1 && e.nout <= 3, -3, "launch_expand3: 1..64 vectors, 1..3 outputs");
+    hipLaunchKernelGGL(k_expand3, dim3(nblocks(n)), dim3(BS), 0, s, e, n);
+    TT_HIP(hipGetLastError());
+}
+
 template <int NQ>
 __global__ void k_proj3(const double* __restrict__ Q, int k0, const double* __restrict__ a,
                         const double* __restrict__ b, const double* __restrict__ c, long n, double* part) {

@@ -71,33 +71,26 @@ static void dense_mv(int n, const std::vector<double>& X, const double* w, doubl
 }
 
 void Context::expand(const double* coef, double* out) {
-    // up to 8 terms per pass; later passes carry the partial sum as their first term
-    const long nd = g.nd;
-    int k = 0;
-    bool first = true;
-    while (k < nbasis) {
-        Extrap e{};
-        int m = 0;
-        if (!first) {
-            e.v[0] = out;
-            e.c[0] = 1.0;
-            m = 1;
-        }
-        while (m < 8 && k < nbasis) {
-            e.v[m] = Qb.p + (size_t)k * nd;
-            e.c[m] = coef[k];
-            m++;
-            k++;
-        }
-        e.m = m;
-        launch_extrapolate(e, out, nd, stream);
-        first = false;
-    }
+    Expand3 e{};
+    e.Q = Qb.p;
+    e.nb = nbasis;
+    e.nout = 1;
+    e.out[0] = out;
+    for (int k = 0; k < nbasis; k++) e.c[0][k] = coef[k];
+    launch_expand3(e, g.nd, stream);
 }
 
 void Context::expand_state(int t, double* out) {
-    const long nd = g.nd;
-    for (int c = 0; c < 3; c++) expand(rst.data() + ((size_t)t * 3 + c) * nbasis, out + (size_t)c * nd);
+    // u, v and a in one pass over the basis
+    Expand3 e{};
+    e.Q = Qb.p;
+    e.nb = nbasis;
+    e.nout = 3;
+    for (int c = 0; c < 3; c++) {
+        e.out[c] = out + (size_t)c * g.nd;
+        for (int k = 0; k < nbasis; k++) e.c[c][k] = rst[((size_t)t * 3 + c) * nbasis + k];
+    }
+    launch_expand3(e, g.nd, stream);
 }
 
 void Context::project3(const double* h0, const double* h1, const double* h2, double* out) {
