@@ -19,7 +19,8 @@
  *     the initial displacement from -restartFileVecSol as the reference does;
  *   - the reduced-order path (reduction = TRUE, SOMMG) runs with the reduced systems solved directly,
  *     as in the reference; it reads and writes no solution restart file (RestartSol0x.dat);
- *   - numerical non-convergence and non-finite values are reported as errors (< 0).
+ *   - numerical non-convergence and non-finite values are reported as errors (< 0);
+ *   - option names match without regard to case, as in PETSc's options database.
  */
 #ifndef TOPOPT_DROPIN_H
 #define TOPOPT_DROPIN_H
@@ -100,6 +101,19 @@ PetscErrorCode DMDestroy(DM* dm);
 
 /* The tt_* context shared by the classes built on one mesh (and the one MMA uses). */
 tt_ctx* topopt_dropin_context(DM da_nodes);
+
+/* The tt_params the LinearElasticity constructor (LinearElasticity.cc:14-120) creates its context with,
+   from the constructor's arguments and the options database: the reference's time grid (dt = t1/(Ts-1),
+   :96-99), load (location presets 0/1, Gaussian-cosine signal; -loadsignal is not read, as in the
+   reference) and outer-solver tolerances (rtol 1e-10, 200 iterations, :2750-2754, or -ksp_rtol /
+   -ksp_max_it), plus this library's own options (-precision, -slots, -cheb_degree, -filter_rtol, and the
+   y-line half-sine pulse preset -tt_loadloc 2 -tt_loadsignal 1 -tt_pulse_steps n).  No GPU work; returns
+   nonzero for a load location without a preset. */
+PetscErrorCode LinearElasticityParams(DM da_nodes, PetscScalar nu, PetscInt nlvls, PetscInt ncp, PetscInt tsc,
+                                      PetscScalar t1, PetscInt ti, PetscInt objective, PetscInt sInt, PetscInt mInt,
+                                      PetscScalar t0, PetscInt boundaries, PetscInt loadloc, PetscBool reduction,
+                                      PetscInt basis, PetscScalar dalpha, PetscScalar dbeta, PetscScalar omega0,
+                                      PetscInt optimizationRegion, tt_params* out);
 
 class MMA;
 

@@ -1,8 +1,9 @@
 """The main.cc-order design loop through the drop-in classes (topopt_gpu driver, GPU).
 
 Runs the 32x16x16 cantilever design loop (dt 0.01, 100 steps, 3-level multigrid, PDE
-filter rmin 0.08 with the reference's default projection, volume fraction 0.12, 5 MMA
-iterations) and compares its per-iteration history and final design with the oracle's
+filter rmin 0.08 with the reference's default projection, volume fraction and initial design
+0.12 (-volfrac 0.12 -V0 0.12), the half-sine y-line pulse (-tt_loadloc 2 -tt_loadsignal 1
+-tt_pulse_steps 20), 5 MMA iterations) and compares its per-iteration history and final design with the oracle's
 run of the same loop (tests/golden/design_loop_32x16x16.npz), both with solves at rtol
 1e-12.  The design loop amplifies solver round-off a little each iteration, so the bar is
 1e-8 relative on the objective history and 1e-7 on the final densities (FP64), 1e-5 and
@@ -24,11 +25,11 @@ DRIVER = os.path.join(ROOT, "topopt_in_petsc_transient_amd", "topopt_gpu")
 GOLD = os.path.join(ROOT, "tests", "golden", "design_loop_32x16x16.npz")
 
 ARGS = ["-nx", "33", "-ny", "17", "-nz", "17", "-xcmax", "2", "-ycmax", "1", "-zcmax", "1", "-nlvls", "3",
-        "-boundaries", "1", "-loadloc", "2", "-loadsignal", "1", "-pulse_steps", "20", "-optimizationRegion", "2",
+        "-boundaries", "1", "-tt_loadloc", "2", "-tt_loadsignal", "1", "-tt_pulse_steps", "20", "-optimizationRegion", "2",
         "-objective", "3", "-Emax", "1", "-Ef", "1e-9", "-Rmax", "1", "-Rf", "1e-3", "-penalK", "3", "-penalM", "1",
         "-sInt", "1", "-massInterpolation", "0", "-dalph", "1e-2", "-dbeta", "1e-4", "-nu", "0.3", "-ti", "1",
         "-t1", "1.0", "-ncp", "1", "-tsc", "100", "-filter", "2", "-rmin", "0.08", "-projectionFilter", "1",
-        "-beta", "0.1", "-eta", "0.15", "-volfrac", "0.12", "-movlim", "0.2", "-maxItr", "5",
+        "-beta", "0.1", "-eta", "0.15", "-volfrac", "0.12", "-V0", "0.12", "-movlim", "0.2", "-maxItr", "5",
         "-ksp_rtol", "1e-12", "-ksp_max_it", "500", "-filter_rtol", "1e-13"]
 
 

@@ -150,6 +150,31 @@ def test_initial_displacement_matches_the_oracle():
     assert out[4][0] == out[0][0] and np.array_equal(out[4][1], out[0][1])
 
 
+def test_initial_displacement_before_the_density():
+    """tt_set_initial_displacement right after tt_create, before any tt_set_density (the drop-in constructor
+    calls it there): N.u0 comes from the node mask alone, so state 0 is exactly N.u0 and the run is
+    bit-identical to one that sets u0 after the density."""
+    po, pg = problem(nsteps=10)
+    n = pg.nelx * pg.nely * pg.nelz
+    x = random_density(n, seed=7)
+    u0 = np.random.default_rng(8).standard_normal(3 * O.nn(po)) * 1e-2
+    res = []
+    for before in (True, False):
+        c = capi.Context(pg)
+        if before:
+            c.set_initial_displacement(u0)
+        c.set_density(x)
+        if not before:
+            c.set_initial_displacement(u0)
+        c.complete_initial_state()
+        c.forward()
+        u_0 = c.get_state(0)[0]
+        assert np.array_equal(u_0, u0 * O.bc_mask(po))
+        res.append(c.objective_gradient())
+        c.close()
+    assert res[0][0] == res[1][0] and np.array_equal(res[0][1], res[1][1])
+
+
 def test_driver_reads_the_initial_displacement(tmp_path):
     """-onlyLoadDesign -solutionAsU0 -restartFileVecSol f: the driver's first objective is the library's
     with that initial displacement on the first design."""

@@ -17,6 +17,14 @@
 #include <map>
 
 // ------------------------------------------------------------------ minimal PETSc names
+// The options database matches names without regard to case, as PETSc's does (the reference's job
+// scripts pass -Rmin for the -rmin that TopOpt.cc:363 reads): names are stored and looked up folded
+// to lower case.  Values keep their case.
+static std::string fold(const char* name) {
+    std::string s(name);
+    for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
+    return s;
+}
 static std::map<std::string, std::string>& options() {
     static std::map<std::string, std::string> db;
     return db;
@@ -30,7 +38,7 @@ PetscErrorCode PetscInitialize(int* argc, char*** argv, const char*, const char*
         const bool has_val = i + 1 < *argc && ((*argv)[i + 1][0] != '-' || ((*argv)[i + 1][1] >= '0' &&
                                                                              (*argv)[i + 1][1] <= '9') ||
                                               (*argv)[i + 1][1] == '.');
-        db[a] = has_val ? (*argv)[++i] : "1";
+        db[fold(a)] = has_val ? (*argv)[++i] : "1";
     }
     return 0;
 }
@@ -44,7 +52,7 @@ PetscErrorCode PetscPrintf(int, const char* fmt, ...) {
     return 0;
 }
 static const std::string* opt(const char* name) {
-    auto it = options().find(name);
+    auto it = options().find(fold(name));
     return it == options().end() ? nullptr : &it->second;
 }
 PetscErrorCode PetscOptionsGetInt(void*, const char*, const char* name, PetscInt* v, PetscBool* set) {
@@ -61,7 +69,10 @@ PetscErrorCode PetscOptionsGetReal(void*, const char*, const char* name, PetscRe
 }
 PetscErrorCode PetscOptionsGetBool(void*, const char*, const char* name, PetscBool* v, PetscBool* set) {
     const std::string* s = opt(name);
-    if (s) *v = (*s == "0" || *s == "false" || *s == "FALSE" || *s == "no") ? PETSC_FALSE : PETSC_TRUE;
+    if (s) {
+        const std::string b = fold(s->c_str());
+        *v = (b == "0" || b == "false" || b == "no" || b == "off") ? PETSC_FALSE : PETSC_TRUE;
+    }
     if (set) *set = s ? PETSC_TRUE : PETSC_FALSE;
     return 0;
 }
@@ -401,8 +412,9 @@ PetscErrorCode TopOpt::SetUp() {
     PetscOptionsGetReal(NULL, NULL, "-beta", &beta, &flg);
     PetscOptionsGetReal(NULL, NULL, "-betaFinal", &betaFinal, &flg);
     PetscOptionsGetReal(NULL, NULL, "-eta", &eta, &flg);
+    // V0 keeps the default volume fraction it was given before -volfrac is read (TopOpt.cc:113-114, 373);
+    // only -V0 changes it
     PetscOptionsGetReal(NULL, NULL, "-volfrac", &volfrac, &flg);
-    V0 = volfrac;
     PetscOptionsGetReal(NULL, NULL, "-Xmax", &Xmax, &flg);
     PetscOptionsGetReal(NULL, NULL, "-Xmin", &Xmin, &flg);
     PetscOptionsGetInt(NULL, NULL, "-maxItr", &maxItr, &flg);
@@ -571,6 +583,65 @@ PetscErrorCode TopOpt::WriteRestartFiles(PetscInt* itr, MMA* mma) {
 }
 
 // ------------------------------------------------------------------ LinearElasticity
+PetscErrorCode LinearElasticityParams(DM da_nodes, PetscScalar nu_in, PetscInt nlvls_in, PetscInt ncp_in,
+                                      PetscInt tsc_in, PetscScalar t1_in, PetscInt ti_in, PetscInt objective_in,
+                                      PetscInt sInt_in, PetscInt mInt_in, PetscScalar, PetscInt boundaries_in,
+                                      PetscInt loadloc_in, PetscBool reduction_in, PetscInt basis_in,
+                                      PetscScalar dalpha_in, PetscScalar dbeta_in, PetscScalar omega0_in,
+                                      PetscInt optimizationRegion, tt_params* out) {
+    tt_params p;
+    tt_default_params(&p);
+    // SOMMG reduced-order model (LinearElasticity.cc:404-461, 506-508, 994-1003)
+    p.reduction = reduction_in ? 1 : 0;
+    p.basis = basis_in;
+    p.omega0 = omega0_in;
+    p.nelx = da_nodes->nxyz[0] - 1;
+    p.nely = da_nodes->nxyz[1] - 1;
+    p.nelz = da_nodes->nxyz[2] - 1;
+    for (int i = 0; i < 6; i++) p.xc[i] = da_nodes->xc[i];
+    p.nu = nu_in;
+    p.nlvls = nlvls_in;
+    p.nsteps = ncp_in * tsc_in;  // Ts - 1 (LinearElasticity.cc:96)
+    p.t1 = t1_in;                // dt = t1 / (Ts - 1) (:99); t0 takes no part in the time grid or the load
+    p.ti = ti_in;
+    p.objective = objective_in;
+    p.sInt = sInt_in;
+    p.mInt = mInt_in;
+    p.boundaries = boundaries_in;
+    p.region = optimizationRegion;
+    p.dalpha = dalpha_in;
+    p.dbeta = dbeta_in;
+    // The reference's load: the location presets 0 and 1 of LoadLocation (:616-675) with the Gaussian-cosine
+    // signal of Load (:593-615), which reads no option (the -loadsignal its job scripts pass is ignored).
+    // The half-sine pulse on the y-line at x = xmax, z = zmin is this library's own preset, chosen with
+    // options the reference does not know: -tt_loadloc 2 -tt_loadsignal 1 -tt_pulse_steps n.
+    p.loadloc = loadloc_in;
+    p.loadsignal = 0;
+    PetscBool flg, own = PETSC_FALSE;
+    PetscOptionsGetInt(NULL, NULL, "-tt_loadloc", &p.loadloc, &own);
+    PetscOptionsGetInt(NULL, NULL, "-tt_loadsignal", &p.loadsignal, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-tt_pulse_steps", &p.pulse_steps, &flg);
+    if (own ? (p.loadloc < 0 || p.loadloc > 2) : (p.loadloc != 0 && p.loadloc != 1)) {
+        PetscPrintf(PETSC_COMM_WORLD, "# load location %d: the reference's presets are -loadloc 0 and 1 "
+                    "(LinearElasticity.cc:627-656); -tt_loadloc 2 selects the y-line at x = xmax, z = zmin\n",
+                    p.loadloc);
+        return TT_ERR_ARG;
+    }
+    // the outer solver: the reference's fine-level KSP tolerances (rtol 1e-10, 200 iterations,
+    // LinearElasticity.cc:2750-2754), changed by -ksp_rtol / -ksp_max_it as KSPSetFromOptions (:2794) does
+    p.rtol = 1e-10;
+    p.maxit = 200;
+    PetscOptionsGetReal(NULL, NULL, "-ksp_rtol", &p.rtol, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-ksp_max_it", &p.maxit, &flg);
+    // options with no reference counterpart: precision, state slots, smoother degree, filter solve
+    PetscOptionsGetInt(NULL, NULL, "-precision", &p.precision, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-slots", &p.checkpoint_slots, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-cheb_degree", &p.cheb_degree, &flg);
+    PetscOptionsGetReal(NULL, NULL, "-filter_rtol", &p.filter_rtol, &flg);
+    *out = p;
+    return 0;
+}
+
 LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt nlvls_in, PetscInt ncp_in,
                                    PetscInt tsc_in, PetscScalar t1_in, PetscInt ti_in, PetscInt objective_in,
                                    PetscInt sInt_in, PetscInt mInt_in, PetscScalar t0_in, PetscInt boundaries_in,
@@ -586,39 +657,10 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     ncp = ncp_in;
     reduced = reduction_in;
     tt_params p;
-    tt_default_params(&p);
-    // SOMMG reduced-order model (LinearElasticity.cc:404-461, 506-508, 994-1003)
-    p.reduction = reduction_in ? 1 : 0;
-    p.basis = basis_in;
-    p.omega0 = omega0_in;
-    p.nelx = da_nodes->nxyz[0] - 1;
-    p.nely = da_nodes->nxyz[1] - 1;
-    p.nelz = da_nodes->nxyz[2] - 1;
-    for (int i = 0; i < 6; i++) p.xc[i] = da_nodes->xc[i];
-    p.nu = nu_in;
-    p.nlvls = nlvls_in;
-    p.nsteps = Ts - 1;
-    p.t1 = t1_in - t0_in;
-    p.ti = ti_in;
-    p.objective = objective_in;
-    p.sInt = sInt_in;
-    p.mInt = mInt_in;
-    p.boundaries = boundaries_in;
-    p.loadloc = loadloc_in;
-    p.region = optimizationRegion;
-    p.dalpha = dalpha_in;
-    p.dbeta = dbeta_in;
-    // options with no reference counterpart: precision, state slots, solver, load signal
-    PetscBool flg;
-    PetscOptionsGetInt(NULL, NULL, "-precision", &p.precision, &flg);
-    PetscOptionsGetInt(NULL, NULL, "-slots", &p.checkpoint_slots, &flg);
-    PetscOptionsGetReal(NULL, NULL, "-ksp_rtol", &p.rtol, &flg);
-    PetscOptionsGetInt(NULL, NULL, "-ksp_max_it", &p.maxit, &flg);
-    PetscOptionsGetInt(NULL, NULL, "-cheb_degree", &p.cheb_degree, &flg);
-    p.loadsignal = 0;  // the reference's Gaussian-cosine signal (LinearElasticity.cc:593-615) by default
-    PetscOptionsGetInt(NULL, NULL, "-loadsignal", &p.loadsignal, &flg);
-    PetscOptionsGetInt(NULL, NULL, "-pulse_steps", &p.pulse_steps, &flg);
-    PetscOptionsGetReal(NULL, NULL, "-filter_rtol", &p.filter_rtol, &flg);
+    if (LinearElasticityParams(da_nodes, nu_in, nlvls_in, ncp_in, tsc_in, t1_in, ti_in, objective_in, sInt_in,
+                               mInt_in, t0_in, boundaries_in, loadloc_in, reduction_in, basis_in, dalpha_in,
+                               dbeta_in, omega0_in, optimizationRegion, &p) != 0)
+        std::exit(1);
     if (report(nullptr, tt_create(&p, &ctx), "tt_create") != TT_OK) std::exit(1);
     if (!da_nodes->mesh) da_nodes->mesh = std::make_shared<TTMesh>();
     if (da_nodes->mesh->ctx) tt_destroy(da_nodes->mesh->ctx);  // a filter-only context is superseded
@@ -629,7 +671,7 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     // starting guesses of the initial-acceleration, terminal-adjoint and t = 0 adjoint solves
     restart = PETSC_TRUE;
     flip = PETSC_TRUE;
-    PetscBool onlyDesign = PETSC_FALSE, solutionAsU0 = PETSC_FALSE;
+    PetscBool flg, onlyDesign = PETSC_FALSE, solutionAsU0 = PETSC_FALSE;
     PetscOptionsGetBool(NULL, NULL, "-restart", &restart, &flg);
     PetscOptionsGetBool(NULL, NULL, "-onlyLoadDesign", &onlyDesign, &flg);
     PetscOptionsGetBool(NULL, NULL, "-solutionAsU0", &solutionAsU0, &flg);

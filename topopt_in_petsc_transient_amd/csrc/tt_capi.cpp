@@ -796,9 +796,8 @@ int tt_set_initial_displacement(tt_ctx* ctx, const double* u0, int on_device) {
         if (!C.u0in.p) C.u0in.alloc(nd);
         TT_HIP(hipMemcpyAsync(C.u0in.p, u0, nd * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               C.stream));
-        // N.u0 (:584): the boundary-condition form with zero coefficients returns (1 - N).u0
-        tt::launch_apply_bc(C.op_fine64, 0.0, 0.0, C.u0in.p, C.rhs.p, true, C.stream);
-        tt::launch_lincomb(1.0, C.u0in.p, -1.0, C.rhs.p, C.u0in.p, nd, C.stream);
+        // N.u0 (:584), from the node mask alone (no operator apply: valid before any tt_set_density)
+        tt::launch_mask_dofs(C.u0in.p, C.mask.p, nd, C.stream);
         C.u0_set = true;
         C.sync();
     }

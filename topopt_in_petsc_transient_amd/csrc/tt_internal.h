@@ -326,6 +326,8 @@ void launch_mma_change(const double* x, double* xold, long n, Reducer& red, int 
 void launch_mma_kkt(const double* x, const double* df, const double* dg, const double* xmin, const double* xmax,
                     long n, double lam, Reducer& red, hipStream_t s);
 void launch_fill_scal(void* x, long n, double v, hipStream_t s);
+// x = N.x (clamped dofs of the node mask set to 0)
+void launch_mask_dofs(double* x, const uint8_t* mask, long nd, hipStream_t s);
 void launch_scale(double* x, long n, double a, hipStream_t s);
 // out = a*x + b*y (out may alias x or y)
 void launch_lincomb(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);

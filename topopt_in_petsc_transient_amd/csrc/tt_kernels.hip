@@ -914,6 +914,17 @@ void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s) {
 }
 void launch_fill_scal(void* x, long n, double v, hipStream_t s) { launch_fill(x, v, n, true, s); }
 
+// x = N.x: dof c of node n is zeroed where bit c of mask[n] is set
+__global__ void k_mask_dofs(double* __restrict__ x, const uint8_t* __restrict__ mask, long nd) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < nd && ((mask[i / 3] >> (i % 3)) & 1)) x[i] = 0.0;
+}
+void launch_mask_dofs(double* x, const uint8_t* mask, long nd, hipStream_t s) {
+    if (nd <= 0) return;
+    hipLaunchKernelGGL(k_mask_dofs, dim3(nblocks(nd)), dim3(BS), 0, s, x, mask, nd);
+    TT_HIP(hipGetLastError());
+}
+
 template <typename S, typename D>
 __global__ void k_copy(const S* __restrict__ a, D* __restrict__ b, long n) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
