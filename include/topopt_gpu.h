@@ -221,6 +221,18 @@ int tt_petsc_read_vecs(const char* path, int nvec, double* const* vecs, const lo
    average milliseconds per launch.  u and y are device arrays (FP64 or FP32 per prec). */
 int tt_time_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int nrep, double* ms_per_apply);
 
+/* No counterpart in the reference: the allocation-free form of tt_volume_constraint
+   (LinearElasticity.cc:1283-1296), for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops).  No context: xphys (nel doubles), gx (one double) and dgdx (nel doubles,
+   may be NULL) are device arrays on the current device; the work is queued on `stream` (a hipStream_t;
+   NULL = the null stream) and the call returns without synchronising.  The block partial sums live in
+   a library-owned device array of 256 doubles (2 KB), which each GPU the library runs on holds until the
+   process ends; calls that overlap on one GPU race on it.  No hipMalloc, hipFree or allocating library
+   call.  1 <= nel <= 2^31-1, else TT_ERR_ARG; tt_last_error(NULL) gives the message.  gx is bit-identical
+   to tt_volume_constraint's. */
+int tt_volume_constraint_noalloc(const double* xphys, long nel, double volfrac, double* gx, double* dgdx,
+                                 void* stream);
+
 int tt_get_stats(tt_ctx* ctx, tt_stats* st);
 int tt_reset_stats(tt_ctx* ctx);
 int tt_synchronize(tt_ctx* ctx);

@@ -28,6 +28,7 @@ EXPORTED = [
     "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
     "tt_mma_get_state", "tt_mma_set_state", "tt_get_restart_vectors", "tt_set_restart_vectors",
     "tt_petsc_write_vecs", "tt_petsc_read_vecs", "tt_get_reduced_model", "tt_set_initial_displacement",
+    "tt_volume_constraint_noalloc",
 ]
 
 
@@ -105,6 +106,7 @@ def lib():
             "tt_petsc_read_vecs": [ctypes.c_char_p, ctypes.c_int, vp, vp],
             "tt_get_reduced_model": [vp, vp, ip, vp, vp, vp, vp, vp, vp],
             "tt_set_initial_displacement": [vp, vp, ip],
+            "tt_volume_constraint_noalloc": [vp, ctypes.c_long, ctypes.c_double, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -143,6 +145,15 @@ def _ptr(a):
     if isinstance(a, np.ndarray):
         return a.ctypes.data, 0
     return a.data_ptr(), int(a.is_cuda)
+
+
+def volume_constraint_noalloc(xphys, volfrac, gx, dgdx=None, stream=0):
+    """tt_volume_constraint_noalloc on device tensors (xphys, gx of one element, dgdx or None), queued
+    on `stream` (a raw hipStream_t, 0 = the null stream); returns without synchronising."""
+    rc = lib().tt_volume_constraint_noalloc(xphys.data_ptr(), xphys.numel(), volfrac, gx.data_ptr(),
+                                            None if dgdx is None else dgdx.data_ptr(), stream or None)
+    if rc != 0:
+        raise TTError(rc, lib().tt_last_error(None).decode())
 
 
 class Context:

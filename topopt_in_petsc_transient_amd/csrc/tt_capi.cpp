@@ -356,6 +356,10 @@ struct tt_ctx {
 };
 
 static thread_local std::string g_create_error;
+namespace tt {
+// message of a failed call made without a context, returned by tt_last_error(NULL)
+void set_contextless_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace tt
 
 #define TT_API_BEGIN(ctxp)                                    \
     if (!(ctxp) || !(ctxp)->c) return TT_ERR_ARG;             \

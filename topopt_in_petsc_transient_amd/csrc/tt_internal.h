@@ -67,6 +67,9 @@ struct DBuf {
     T* get() const { return p; }
 };
 
+// message of a failed call made without a context (tt_last_error(NULL) returns it)
+void set_contextless_error(const std::string& msg);
+
 // ------------------------------------------------------------------ grid
 struct Grid {
     int ex, ey, ez;  // elements
@@ -308,6 +311,9 @@ void launch_project_chain(const double* xt, double* df, double* dg, long n, doub
                           hipStream_t s);
 // reductions over element arrays
 void launch_sum(const double* x, long n, Reducer& red, int slot, hipStream_t s);
+// the same block partial sums of x written to `partials` (room for `capacity` values; at most 256 are
+// needed, whatever n is); returns the number of blocks
+int launch_sum_partials(const double* x, long n, double* partials, int capacity, hipStream_t s);
 void launch_mnd(const double* x, long n, Reducer& red, int slot, hipStream_t s);
 
 // MMA kernels (m = 1)

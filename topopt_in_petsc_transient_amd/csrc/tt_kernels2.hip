@@ -1,5 +1,6 @@
 // HIP kernels for gfx950: element sensitivities and objective, PDE (Helmholtz)
 // filter, Heaviside projection, and the element-wise parts of MMA.
+#include "../../include/topopt_gpu.h"
 #include "tt_internal.h"
 
 namespace tt {
@@ -374,6 +375,13 @@ void launch_sum(const double* x, long n, Reducer& red, int region, hipStream_t s
                        (double*)nullptr);
     set_region(red, region, rblk(n), 1);
     TT_HIP(hipGetLastError());
+}
+int launch_sum_partials(const double* x, long n, double* partials, int capacity, hipStream_t s) {
+    const int nb = rblk(n);
+    TT_CHECK(nb <= capacity, TT_ERR_INTERNAL, "sum: more block partials than the partial buffer holds");
+    hipLaunchKernelGGL(k_sum, dim3(nb), dim3(BS2), 0, s, x, n, 0, partials, (unsigned*)nullptr, (double*)nullptr);
+    TT_HIP(hipGetLastError());
+    return nb;
 }
 void launch_mnd(const double* x, long n, Reducer& red, int region, hipStream_t s) {
     hipLaunchKernelGGL(k_sum, dim3(rblk(n)), dim3(BS2), 0, s, x, n, 1, red.part(region), (unsigned*)nullptr,
