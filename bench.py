@@ -40,6 +40,8 @@ MFMA_EXEC_A32 = 17 * 9 * 17 * 28 * 9 * 16384
 # HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
 # scaled by the counters' reading for a copy kernel of known size); None until measured.
 TRAFFIC_A32 = 20751964  # FETCH 6936.6 KiB x 1.99922 + WRITE 6397.8 KiB x 1.0 (copy-calibrated)
+# the same for the FP64 flexible-CG apply; None until measured
+TRAFFIC_FCG64 = None
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
@@ -99,8 +101,10 @@ def op_microbench(capi, torch, nrep):
             "fp64_ms": ms64, "fp32_ms": ms32, "fp64_applies_per_s": 1e3 / ms64, "fp32_applies_per_s": 1e3 / ms32,
             "fp32_rel_err_vs_fp64": err, "flops": fl, "fp64_tflops": fl / (ms64 * 1e-3) / 1e12,
             "fp32_tflops": fl / (ms32 * 1e-3) / 1e12, "fp64_GBs": b64 / (ms64 * 1e-3) / 1e9,
-            "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9, "nel": nel,
+            "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9, "nel": nel, "nd": nd,
         }
+    # the FP64 flexible-CG apply (the Krylov iteration's outer residual), timed the same way
+    out["fcg_fp64_ms"] = ctx.time_fcg_apply(nrep)
     ctx.close()
     # calibration for the HBM traffic counters: a plain device copy of 64 MiB (64 MiB read + 64 MiB written)
     a = torch.empty(1 << 24, dtype=torch.float32, device=dev).normal_()
@@ -128,16 +132,24 @@ def forward_run(capi, x, precision, nsteps):
 
 
 def gradient_run(capi, x, precision, nsteps, slots):
+    """One design iteration's physics on resident data: tt_complete_initial_state (multigrid set-up for
+    the Newmark operator, coarsest factorisation, the initial-acceleration solve), then the forward
+    sweep (binomial checkpoints in `slots` HBM slots) and the backward adjoint sweep with the
+    sensitivities.  `seconds` is the whole of it; the three parts are reported as well."""
     ctx = capi.Context(params(capi, precision, nsteps, slots=slots))
     ctx.set_density(x)
+    ctx.synchronize()
     t0 = time.perf_counter()
     ctx.complete_initial_state()
+    t1 = time.perf_counter()
     ctx.forward()
+    t2 = time.perf_counter()
     fx, df = ctx.objective_gradient()
-    el = time.perf_counter() - t0
+    t3 = time.perf_counter()
     st = ctx.stats()
     ctx.close()
-    return {"seconds": el, "forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
+    return {"seconds": t3 - t0, "setup_seconds": t1 - t0, "forward_seconds": t2 - t1, "adjoint_seconds": t3 - t2,
+            "steps": nsteps, "slots": slots, "forward_steps": st.forward_steps, "adjoint_steps": st.adjoint_steps,
             "solves": st.solves, "iters_per_solve": st.krylov_iters / max(1, st.solves)}, fx, df
 
 
@@ -218,20 +230,20 @@ def gradient_roofline_s(precision, forward_steps, adjoint_steps, iters):
     return forward_steps * step + adjoint_steps * (step + extra)
 
 
-def cpu_baseline(steps):
-    """Oracle (FP64 C restatement, Jacobi-PCG) on the first `steps` Newmark steps of the same forward sweep."""
+def cpu_baseline(x, steps):
+    """The oracle (the FP64 C restatement of the reference's Newmark sweep, OpenMP) on the first `steps`
+    Newmark steps of the same forward sweep: same mesh, load, dt, density and rtol; its linear solver is
+    Jacobi-preconditioned CG from the previous displacement (the reference's own solver is PETSc's
+    FGMRES with a Galerkin multigrid, which needs PETSc)."""
     from oracle import oracle as O
-    x = None
     p = O.cantilever_params(NEL[0], NEL[1], NEL[2], nsteps=steps, dt=DT, pulse_steps=20, rtol=1e-8)
-    n = O.nel(p)
-    x = np.full(n, 0.12)
     t0 = time.perf_counter()
     O.forward(p, x)
     el = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": steps / el, "unit": "Newmark steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle forward, first {steps} steps of the 128x64x64 sweep (dt 0.005, rtol 1e-8), "
-                      f"uniform density 0.12, Jacobi-PCG, {threads} OpenMP threads"}
+    return {"value": steps / el, "unit": "Newmark steps/s", "cores": threads, "kind": "port", "seconds": el,
+            "sample": f"oracle forward, first {steps} of the 200 steps of the 128x64x64 sweep (dt 0.005, rtol 1e-8, "
+                      f"the same filtered-noise density), Jacobi-preconditioned CG, {threads} OpenMP threads"}
 
 
 def main():
@@ -240,8 +252,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--slots", type=int, default=20)
+    ap.add_argument("--grad-steps", type=int, default=200,
+                    help="time steps of the adjoint-gradient leg (the 200-step, 20-slot gradient workload, "
+                         "whatever --steps is)")
     ap.add_argument("--nrep", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-reduced", action="store_true", help="skip the SOMMG reduced-order measurement")
@@ -292,21 +307,23 @@ def main():
     u_err = float(np.linalg.norm(u_mixed - u_fp64) / max(np.linalg.norm(u_fp64), 1e-300))
     grad = None
     if not args.no_grad:
-        log("adjoint gradient, mixed")
-        gm, fxm, dfm = gradient_run(capi, x, capi.TT_PREC_MIXED, args.steps, args.slots)
+        log(f"adjoint gradient ({args.grad_steps} steps, {args.slots} slots), mixed")
+        gm, fxm, dfm = gradient_run(capi, x, capi.TT_PREC_MIXED, args.grad_steps, args.slots)
         log("adjoint gradient, FP64")
-        gd, fxd, dfd = gradient_run(capi, x, capi.TT_PREC_FP64, args.steps, args.slots)
+        gd, fxd, dfd = gradient_run(capi, x, capi.TT_PREC_FP64, args.grad_steps, args.slots)
         grad = {"mixed": gm, "fp64": gd, "speedup": gd["seconds"] / gm["seconds"],
+                "speedup_sweeps": (gd["forward_seconds"] + gd["adjoint_seconds"]) /
+                                  (gm["forward_seconds"] + gm["adjoint_seconds"]),
                 "fx_rel_err": abs(fxm - fxd) / abs(fxd),
                 "dfdx_rel_err": float(np.linalg.norm(dfm - dfd) / np.linalg.norm(dfd)), "fx": fxd,
-                "slots": args.slots}
+                "steps": args.grad_steps, "slots": args.slots}
     reduced = None
     if grad and not args.no_reduced:
         log("reduced-order model (SOMMG, basis 5), mixed and FP64")
         reduced = {}
         for name, prec in (("mixed", capi.TT_PREC_MIXED), ("fp64", capi.TT_PREC_FP64)):
             try:
-                rr, fxr, dfr = reduced_run(capi, x, prec, args.steps, 5, 0.01)
+                rr, fxr, dfr = reduced_run(capi, x, prec, args.grad_steps, 5, 0.01)
                 rr["fx"] = fxr
                 rr["fx_rel_diff_vs_full"] = abs(fxr - fxd) / abs(fxd)
                 rr["dfdx_rel_diff_vs_full"] = float(np.linalg.norm(dfr - dfd) / np.linalg.norm(dfd))
@@ -319,25 +336,37 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU oracle sample")
-        cpu = cpu_baseline(args.cpu_steps)
+        cpu = cpu_baseline(x, args.cpu_steps)
 
-    # roofline of the dominant kernel: the FP32 Newmark-operator apply of the V-cycle smoother.
-    # Algorithmic work per launch: the stacked [K; M] 48x24 element contraction, 2*1152 flops per element.
+    # roofline of the dominant kernel of the step: the FP64 flexible-CG apply of every Krylov iteration
+    # (p = z + beta p_old formed while staging, q = A p, p.q partials), the largest single kernel of the
+    # mixed step.  Algorithmic work per launch: the stacked [K; M] 48x24 element contraction,
+    # 2 x 1152 flops per element; its algorithmic bytes: z, p_old read, p, q written (FP64), kfac, mfac, mask.
+    fl = 2 * 1152 * ops["A"]["nel"]
+    nd0, nel0 = ops["A"]["nd"], ops["A"]["nel"]
+    fcg_ms = ops["fcg_fp64_ms"]
+    ach = fl / (fcg_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": ach / (PEAK_FP64 / 1e12), "traffic": TRAFFIC_FCG64,
+            "kernel": "k_apply_tiles<double,double,4,true,StageFcg<double>,EpiFcg> (flexible-CG apply: p = z + beta "
+                      "p_old, q = A p, p.q partials; FP64, 128x64x64, 8x8x4-node tiles, v_mfma_f64_16x16x4f64)",
+            "kernel_ms": fcg_ms, "flops_per_launch": fl,
+            "algorithmic_bytes_per_launch": 8 * (4 * nd0 + 2 * nel0) + nd0 // 3}
+    # second entry: the FP32 Newmark-operator apply of the V-cycle smoothers (FP16-split matrix cores)
     a32 = ops["A"]
-    fl = 2 * 1152 * a32["nel"]
-    ach = fl / (a32["fp32_ms"] * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-            "frac": ach / (PEAK_FP32 / 1e12), "traffic": TRAFFIC_A32,
-            "kernel": "k_apply_tiles<float,float,4,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
-                      "128x64x64, 8x8x4-node tiles, FP16-split v_mfma_f32_16x16x32_f16)",
-            "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl,
-            # the FP32-accurate contraction runs as three FP16 products on the matrix cores: executed
-            # matrix-instruction work per launch (2,601 tiles x 28 batches x 9 v_mfma_f32_16x16x32_f16
-            # of 16,384 flops) against the FP16 dense peak
-            "matrix_pipe": {"instr": "v_mfma_f32_16x16x32_f16", "executed_flops_per_launch": MFMA_EXEC_A32,
-                            "achieved": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12,
-                            "unit": "TFLOP/s",
-                            "frac": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / PEAK_FP16}}
+    ach32 = fl / (a32["fp32_ms"] * 1e-3) / 1e12
+    roof["fp32_apply"] = {
+        "bound": "mfma", "achieved": ach32, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+        "frac": ach32 / (PEAK_FP32 / 1e12), "traffic": TRAFFIC_A32,
+        "kernel": "k_apply_tiles<float,float,4,EpiBC> (A u = (1+a2 beta) K u + (a5+a2 alpha) M u, FP32, "
+                  "128x64x64, 8x8x4-node tiles, FP16-split v_mfma_f32_16x16x32_f16)",
+        "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl,
+        # the FP32-accurate contraction runs as three FP16 products on the matrix cores: executed
+        # matrix-instruction work per launch (2,601 tiles x 28 batches x 9 v_mfma_f32_16x16x32_f16
+        # of 16,384 flops) against the FP16 dense peak
+        "matrix_pipe": {"instr": "v_mfma_f32_16x16x32_f16", "executed_flops_per_launch": MFMA_EXEC_A32,
+                        "achieved": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12,
+                        "unit": "TFLOP/s", "frac": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / PEAK_FP16}}
     # whole-step and whole-gradient rooflines (algorithmic work x measured iterations, against the
     # measured times)
     steps_roof = {}

@@ -13,12 +13,15 @@
  * Differences a caller can observe:
  *   - Filter supports the PDE filter only (filterT = 2, Filter.cc:60-119 / 122-206);
  *   - MMA supports one constraint (m = 1);
- *   - restart files (Restart0x.dat, Restart0x_itr_f0.dat, RestartSol0x.dat) are written and read in
- *     PETSc's binary Vec format with the reference's options; the cp%04d.dat checkpoint files are
- *     not written (the binomial checkpoints stay in device memory); -onlyLoadDesign -solutionAsU0 reads
- *     the initial displacement from -restartFileVecSol as the reference does;
+ *   - restart files (Restart0x.dat, Restart0x_itr_f0.dat, RestartSol0x.dat) and the checkpoint files
+ *     cp%04d.dat are written and read in PETSc's binary Vec format with the reference's options and
+ *     names; with ncp = 1 the gradient uses the binomial checkpoints in device memory, with ncp > 1 it
+ *     recomputes each block from the state read back from its file, as the reference does;
+ *     -onlyLoadDesign -solutionAsU0 reads the initial displacement from -restartFileVecSol as the
+ *     reference does;
  *   - the reduced-order path (reduction = TRUE, SOMMG) runs with the reduced systems solved directly,
- *     as in the reference; it reads and writes no solution restart file (RestartSol0x.dat);
+ *     as in the reference; it reads and writes no solution restart file (RestartSol0x.dat) and no
+ *     checkpoint file;
  *   - numerical non-convergence and non-finite values are reported as errors (< 0);
  *   - option names match without regard to case, as in PETSc's options database.
  */
@@ -196,6 +199,10 @@ class LinearElasticity {
     tt_ctx* ctx;
     bool forward_done;
     PetscErrorCode WriteStates(MPIIO* output, Vec xPhys);
+    // checkpoint files cp%04d.dat (LinearElasticity.cc:1128-1137, 1261-1276, 1651-1672)
+    std::string cpdir;
+    std::string CheckpointFile(PetscInt c) const;
+    PetscErrorCode WriteCheckpoint(PetscInt c, PetscInt t);
     // restart (LinearElasticity.cc:2457-2496, 2679-2723): two alternating RestartSol files
     PetscBool restart, flip;
     std::string filename00, filename01;

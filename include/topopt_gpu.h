@@ -76,6 +76,9 @@ typedef struct tt_params {
     int basis;                     /* basis vectors (-basis, 1..64) */
     double omega0;                 /* expansion frequency of the basis solves with K - omega0^2 M (-omega0,
                                       passed unconverted as in LinearElasticity.cc:996) */
+    int coarse_direct_max;         /* largest coarsest multigrid level (dofs) solved through the dense FP64
+                                      inverse; a larger one is smoothed by Chebyshev(12) on D^-1 A instead.
+                                      0 = 6,476 dofs (a 320 MB dense matrix); 1 forces the Chebyshev path */
 } tt_params;
 
 typedef struct tt_stats {
@@ -141,6 +144,18 @@ int tt_set_initial_displacement(tt_ctx* ctx, const double* u0, int on_device);
    reduced-order model the state is the reduced one expanded on the basis.  State 0 is available as soon
    as tt_complete_initial_state has run. */
 int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_device);
+
+/* A state of the forward sweep given by the caller, as ComputeObjectiveSensitivities reads the checkpoint
+   files cp%04d.dat back (LinearElasticity.cc:1651-1672): u, v, a at step t (3*nn values each).  After
+   tt_complete_initial_state, the given states (state 0 is the initial state unless given too) replace the
+   forward sweep: tt_objective_gradient recomputes the states between consecutive given steps, and after
+   the last one up to nsteps, from them (SolveStateBlock, :1298-1397) and runs the adjoint over each such
+   block, from the last block back.  Without binomial checkpointing (checkpoint_slots >= nsteps) every
+   state is kept; otherwise each given state takes a checkpoint slot and the rest recompute within a
+   block.  A state from a file carries no acceleration history, so the first solve after it starts from a
+   constant-acceleration guess.  tt_forward or tt_complete_initial_state drops the given states.
+   TT_ERR_STATE with the reduced-order model. */
+int tt_set_checkpoint(tt_ctx* ctx, int t, const double* u, const double* v, const double* a, int on_device);
 
 /* ComputeObjectiveSensitivities (LinearElasticity.cc:1625-1908): objective fx and dfdx
    (nel) by the backward Newmark adjoint (SolveLagrangen :808-954, SolveLagrange0
@@ -220,6 +235,11 @@ int tt_petsc_read_vecs(const char* path, int nvec, double* const* vecs, const lo
    tt_op_apply kernel on the context's stream, timed with HIP events on that stream; returns the
    average milliseconds per launch.  u and y are device arrays (FP64 or FP32 per prec). */
 int tt_time_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, int nrep, double* ms_per_apply);
+/* Measurement helper with no counterpart in the reference: nrep back-to-back launches of the FP64
+   flexible-CG apply of a Krylov iteration (p = z + beta p_old formed while staging, q = A p with the
+   Newmark operator, p.q partials) on the context's own Krylov vectors, timed with HIP events on the
+   context's stream; average milliseconds per launch. */
+int tt_time_fcg_apply(tt_ctx* ctx, int nrep, double* ms_per_apply);
 
 /* No counterpart in the reference: the allocation-free form of tt_volume_constraint
    (LinearElasticity.cc:1283-1296), for callers that cannot allocate device memory (code under HIP graph

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# TT_ORACLE_LIB: another build of the oracle (the sanitizer test run loads oracle/_asan/liboracle.so)
+_LIB_PATH = os.environ.get("TT_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 
 class Params(ctypes.Structure):

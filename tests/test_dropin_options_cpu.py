@@ -30,10 +30,23 @@ JOBS = "/root/reference/JobScripts"
 
 PROG = r"""
 #include <cstdio>
+#include <string>
 #include "topopt_dropin.h"
 int main(int argc, char** argv) {
     PetscInitialize(&argc, &argv, PETSC_NULL, "");
     TopOpt* opt = new TopOpt();
+    // -test_output f: one design dataset through the field writer (MPIIO), x = 0.25, xTilde = 0.5, xPhys = 0.75
+    char out[PETSC_MAX_PATH_LEN];
+    PetscBool has_out = PETSC_FALSE;
+    PetscOptionsGetString(NULL, NULL, "-test_output", out, sizeof(out), &has_out);
+    if (has_out) {
+        VecSet(opt->x, 0.25);
+        VecSet(opt->xTilde, 0.5);
+        VecSet(opt->xPhys, 0.75);
+        MPIIO* o = new MPIIO(opt->da_nodes, 0, "", 3, "x, xTilde, xPhys", out);
+        if (o->WriteVTK(opt->da_nodes, opt->x, opt->xTilde, opt->xPhys, 7) != 0) return 3;
+        delete o;
+    }
     tt_params p;
     const int rc = LinearElasticityParams(opt->da_nodes, opt->nu, opt->nlvls, opt->ncp, opt->tsc, opt->t1, opt->ti,
                                           opt->objective, opt->sInt, opt->mInt, opt->t0, opt->boundaries,
@@ -63,9 +76,15 @@ def prog(tmp_path_factory):
     src = d / "opts.cpp"
     src.write_text(PROG)
     out = d / "opts"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-O1", "-std=c++17", "-xhip", "-I", os.path.join(ROOT, "include"),
-           str(src), os.path.join(PKG, "csrc", "topopt_classes.cpp"), "-L", PKG, "-ltopopt_gpu",
-           f"-Wl,-rpath,{PKG}", "-o", str(out)]
+    # under the sanitizer test run (tests/test_sanitizers_cpu.py) the program's host code is built with the
+    # sanitizers too and linked against the host-sanitized library
+    san = os.environ.get("TT_SANITIZE_CXXFLAGS", "").split()
+    libdir = os.path.dirname(os.environ["TT_LIB"]) if san and os.environ.get("TT_LIB") else PKG
+    link_san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-Xarch_host",
+                "-shared-libsan"] if san else []
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O1", "-std=c++17", "-xhip", *san, *link_san, "-I",
+           os.path.join(ROOT, "include"), str(src), os.path.join(PKG, "csrc", "topopt_classes.cpp"), "-L", libdir,
+           "-ltopopt_gpu", f"-Wl,-rpath,{libdir}", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     return str(out)
@@ -121,6 +140,25 @@ def test_time_grid_ignores_t0(prog):
 def test_solver_options(prog):
     d = parse(prog, ["-ksp_rtol", "1e-8", "-ksp_max_it", "50"])
     assert (d["p_rtol"], d["p_maxit"]) == (1e-8, 50)
+
+
+def test_field_writer_layout(prog, tmp_path):
+    """The single-rank MPIIO writer (MPIIO.cc:32-586): header, names, points, cells and one design dataset,
+    read back with the layout bin2vtu.py reads."""
+    import numpy as np
+
+    from tests.test_gpu_output import read_output
+    f = tmp_path / "design.dat"
+    parse(prog, ["-nx", "9", "-ny", "5", "-nz", "5", "-nlvls", "2", "-test_output", str(f)])
+    d = read_output(f)
+    assert (d["ndom"], d["npts"], d["ncells"], d["npf"], d["ncf"], d["npe"]) == (1, 225, 128, 0, 3, 8)
+    assert d["cnames"].replace(" ", "") == "x,xTilde,xPhys"
+    assert np.array_equal(d["types"], np.full(128, 12)) and np.array_equal(d["offs"], 8 * np.arange(1, 129))
+    # the default MBB box: 60 x 20 x 20, node (i, j, k) at (7.5 i, 5 j, 5 k)
+    assert np.allclose(d["pts"][[0, 1, 9, 224]], [[0, 0, 0], [7.5, 0, 0], [0, 5, 0], [60, 20, 20]])
+    assert len(d["data"]) == 1 and d["data"][0][0] == 7
+    for field, v in zip(d["data"][0][2], (0.25, 0.5, 0.75)):
+        assert np.all(field == np.float32(v))
 
 
 def test_boolean_values_without_case(prog):

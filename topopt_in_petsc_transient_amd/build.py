@@ -63,5 +63,41 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+# AddressSanitizer + UndefinedBehaviorSanitizer on the host code (the option parser, the PETSc binary
+# reader and writer, the set-up and the C-ABI): -Xarch_host puts the sanitizers on the host compilation
+# only; GPU code is built as usual.  The library leaves the sanitizer runtime unresolved: the process
+# that loads it preloads clang's shared runtime (tests/test_sanitizers_cpu.py).
+SAN_HOST_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+                  "-Xarch_host", "-g"]
+
+
+def build_sanitized(out_dir, jobs=None):
+    """The library with sanitized host code, as out_dir/libtopopt_gpu.so (not the product library)."""
+    os.makedirs(out_dir, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+
+    def comp(src):
+        obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *CXX_FLAGS, *SAN_HOST_FLAGS, "-I", os.path.join(HERE, "..", "include"),
+               "-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd.insert(1, "-xhip")
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        return obj
+
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(comp, LIB_SOURCES))
+    lib = os.path.join(out_dir, "libtopopt_gpu.so")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs, "-L/opt/rocm/lib", "-lrocsolver",
+           "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return lib
+
+
 if __name__ == "__main__":
     build(verbose=True)

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtopopt_gpu.so")
+# TT_LIB: another build of the same library (the sanitizer test run loads the host-sanitized build)
+LIB_PATH = os.environ.get("TT_LIB") or os.path.join(_HERE, "libtopopt_gpu.so")
 
 TT_PREC_FP64 = 0
 TT_PREC_MIXED = 1
@@ -28,7 +29,7 @@ EXPORTED = [
     "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
     "tt_mma_get_state", "tt_mma_set_state", "tt_get_restart_vectors", "tt_set_restart_vectors",
     "tt_petsc_write_vecs", "tt_petsc_read_vecs", "tt_get_reduced_model", "tt_set_initial_displacement",
-    "tt_volume_constraint_noalloc",
+    "tt_volume_constraint_noalloc", "tt_time_fcg_apply", "tt_set_checkpoint",
 ]
 
 
@@ -48,6 +49,7 @@ class Params(ctypes.Structure):
         ("rmin", ctypes.c_double), ("filter_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
         ("graphs", ctypes.c_int),
         ("reduction", ctypes.c_int), ("basis", ctypes.c_int), ("omega0", ctypes.c_double),
+        ("coarse_direct_max", ctypes.c_int),
     ]
 
 
@@ -107,6 +109,8 @@ def lib():
             "tt_get_reduced_model": [vp, vp, ip, vp, vp, vp, vp, vp, vp],
             "tt_set_initial_displacement": [vp, vp, ip],
             "tt_volume_constraint_noalloc": [vp, ctypes.c_long, ctypes.c_double, vp, vp, vp],
+            "tt_time_fcg_apply": [vp, ip, dp],
+            "tt_set_checkpoint": [vp, ip, vp, vp, vp, ip],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -204,16 +208,30 @@ class Context:
         self._chk(lib().tt_time_op_apply(self.h, which, prec, pu, py, nrep, ctypes.byref(ms)))
         return ms.value
 
+    def time_fcg_apply(self, nrep):
+        ms = ctypes.c_double()
+        self._chk(lib().tt_time_fcg_apply(self.h, nrep, ctypes.byref(ms)))
+        return ms.value
+
     def complete_initial_state(self):
         self._chk(lib().tt_complete_initial_state(self.h))
 
     def forward(self):
         self._chk(lib().tt_forward(self.h))
 
-    def get_state(self, t):
+    def get_state(self, t, out=None):
+        """(u, v, a) of step t as host arrays, or into `out`: three FP64 device tensors."""
+        if out is not None:
+            self._chk(lib().tt_get_state(self.h, t, *(o.data_ptr() for o in out), 1))
+            return out
         u, v, a = (np.zeros(self.nd) for _ in range(3))
         self._chk(lib().tt_get_state(self.h, t, u.ctypes.data, v.ctypes.data, a.ctypes.data, 0))
         return u, v, a
+
+    def set_checkpoint(self, t, u, v, a):
+        """State of step t given in place of the forward sweep (host arrays of nd values)."""
+        arrs = [np.ascontiguousarray(q, dtype=np.float64) for q in (u, v, a)]
+        self._chk(lib().tt_set_checkpoint(self.h, t, *(q.ctypes.data for q in arrs), 0))
 
     def objective_gradient(self, dfdx=None):
         out = np.zeros(self.nel) if dfdx is None else dfdx

@@ -696,6 +696,7 @@ LinearElasticity::LinearElasticity(DM da_nodes, DM, PetscScalar nu_in, PetscInt 
     const std::string wd = option_string("-workdir", "./");
     filename00 = wd + "/RestartSol00.dat";
     filename01 = wd + "/RestartSol01.dat";
+    cpdir = wd + "/";
     if (restart && !onlyDesign && reduced) {
         // the reduced systems are solved directly (SetUpSolver2, :3007-3014): no starting guesses to read
         PetscPrintf(PETSC_COMM_WORLD, "# Reduced-order model: -restartFileVecSol is not read (direct reduced solves)\n");
@@ -743,6 +744,10 @@ PetscErrorCode LinearElasticity::CompleteInitialState(Vec xPhys, PetscScalar Emi
     TTCALL(ctx, tt_set_density(ctx, xPhys->a.data(), 0));
     TTCALL(ctx, tt_complete_initial_state(ctx));
     forward_done = false;
+    if (!reduced) {  // the checkpoint file of the initial state (:1128-1137)
+        ierr = WriteCheckpoint(0, 0);
+        if (ierr) return ierr;
+    }
     if (write && output) {  // the initial state (:1086-1113): u0 (0 unless -solutionAsU0 gave one)
         Vec U0;
         VecCreateSeq(0, 3 * (PetscInt)tt_num_nodes(ctx), &U0);
@@ -768,11 +773,37 @@ PetscErrorCode LinearElasticity::WriteStates(MPIIO* output, Vec xPhys) {
     return ierr;
 }
 
-// LinearElasticity.cc:1208-1280 (binomial checkpoints in device memory instead of files)
+// cp%04d.dat (LinearElasticity.cc:1128-1137, 1261-1276): U, DU, DDU of step t as PETSc binary vectors in
+// the work directory
+std::string LinearElasticity::CheckpointFile(PetscInt c) const {
+    char name[32];
+    std::snprintf(name, sizeof(name), "cp%04d.dat", (unsigned short)c);
+    return cpdir + name;
+}
+PetscErrorCode LinearElasticity::WriteCheckpoint(PetscInt c, PetscInt t) {
+    const long nd = 3 * tt_num_nodes(ctx);
+    std::vector<double> u(nd), v(nd), a(nd);
+    TTCALL(ctx, tt_get_state(ctx, t, u.data(), v.data(), a.data(), 0));
+    const double* vs[3] = {u.data(), v.data(), a.data()};
+    const long lens[3] = {nd, nd, nd};
+    const std::string f = CheckpointFile(c);
+    return report(nullptr, tt_petsc_write_vecs(f.c_str(), 3, vs, lens), f.c_str());
+}
+
+// LinearElasticity.cc:1208-1280.  The sweep keeps its binomial checkpoints in device memory; as in the
+// reference, the state at the end of each of the ncp blocks is written to cp%04d.dat (block c + 1), and
+// with ncp > 1 the states are written to the field output as the sweep goes (:1222-1251).
 PetscErrorCode LinearElasticity::GenerateCheckPoints(PetscBool write, Vec xPhys, MPIIO* output) {
     TTCALL(ctx, tt_forward(ctx));
     forward_done = true;
-    if (write && output) return WriteStates(output, xPhys);
+    if (!reduced) {
+        const PetscInt tsc = (Ts - 1) / ncp;
+        for (PetscInt c = 0; c < ncp; c++) {
+            PetscErrorCode ierr = WriteCheckpoint(c + 1, tsc * (c + 1));
+            if (ierr) return ierr;
+        }
+    }
+    if (write && ncp > 1 && output) return WriteStates(output, xPhys);
     return 0;
 }
 
@@ -788,7 +819,21 @@ PetscErrorCode LinearElasticity::ComputeObjectiveSensitivities(PetscScalar* fx, 
                                                    PETSC_FALSE, nullptr);
         if (ierr) return ierr;
     }
-    if (!forward_done) {
+    if (ncp > 1 && !reduced) {
+        // as the reference (:1651-1680): the block start states are read back from cp%04d.dat, blocks
+        // c = ncp-1 .. 0, and the gradient recomputes each block from its file state
+        const long nd = 3 * tt_num_nodes(ctx);
+        const PetscInt tsc = (Ts - 1) / ncp;
+        std::vector<double> u(nd), v(nd), a(nd);
+        double* vs[3] = {u.data(), v.data(), a.data()};
+        const long lens[3] = {nd, nd, nd};
+        for (PetscInt c = ncp - 1; c >= 0; c--) {
+            const std::string f = CheckpointFile(c);
+            if (report(nullptr, tt_petsc_read_vecs(f.c_str(), 3, vs, lens), f.c_str()) != TT_OK) return TT_ERR_ARG;
+            TTCALL(ctx, tt_set_checkpoint(ctx, tsc * c, u.data(), v.data(), a.data(), 0));
+        }
+        forward_done = true;
+    } else if (!forward_done) {
         TTCALL(ctx, tt_forward(ctx));
         forward_done = true;
     }

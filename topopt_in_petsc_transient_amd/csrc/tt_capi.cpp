@@ -19,6 +19,7 @@ static void check_params(const tt_params& p) {
     TT_CHECK(p.precision == TT_PREC_FP64 || p.precision == TT_PREC_MIXED, TT_ERR_ARG, "bad precision mode");
     TT_CHECK(p.cheb_degree >= 1 && p.cheb_degree <= 16, TT_ERR_ARG, "cheb_degree must be 1..16");
     TT_CHECK(p.rtol > 0.0 && p.maxit > 0, TT_ERR_ARG, "need rtol > 0 and maxit > 0");
+    TT_CHECK(p.coarse_direct_max >= 0, TT_ERR_ARG, "coarse_direct_max must be >= 0");
     TT_CHECK(p.loadsignal != 1 || p.pulse_steps > 0, TT_ERR_ARG, "pulse_steps must be positive");
     TT_CHECK(!p.reduction || (p.basis >= 1 && p.basis <= 64), TT_ERR_ARG, "basis must be 1..64");
     TT_CHECK(!p.reduction || p.omega0 != 0.0, TT_ERR_ARG,
@@ -156,6 +157,9 @@ Context::Context(const tt_params& p) : prm(p) {
     op_fine64.mask = mask.p;
     op_fine64.tbc4 = tbc4.p;
     op_fine64.tbc2 = tbc2.p;
+    build_batch_tables(g, btab);
+    op_fine64.tab4 = btab.t4;
+    op_fine64.tab2 = btab.t2;
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
 
@@ -437,6 +441,7 @@ int tt_default_params(tt_params* p) {
     p->verbose = 0;
     p->graphs = 0;
     p->reduction = 0; p->basis = 5; p->omega0 = 1.0;  // TopOpt.cc:157-159
+    p->coarse_direct_max = 0;
     return TT_OK;
 }
 
@@ -558,6 +563,34 @@ int tt_time_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, i
     TT_API_END
 }
 
+int tt_time_fcg_apply(tt_ctx* ctx, int nrep, double* ms) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(C.density_set, TT_ERR_STATE, "tt_set_density must be called first");
+    TT_CHECK(ms && nrep > 0, TT_ERR_ARG, "bad tt_time_fcg_apply arguments");
+    const long nd = C.g.nd;
+    // the flexible-CG apply of a Krylov iteration on the context's own vectors: z and p_old from a
+    // deterministic hash, beta = 0.5, then nrep launches of the tile kernel (the beta kernel excluded)
+    tt::launch_fill_hash(C.kz.p, nd, 0x51ED270B27A1F00Dull, true, C.stream);
+    tt::launch_fill_hash(C.kp0.p, nd, 0x2F1C3A5B4D6E7F81ull, true, C.stream);
+    tt::launch_fill(C.red.scal.p + tt::SC_BETA, 0.5, 1, true, C.stream);
+    tt::launch_fcg_apply(C.op_fine64, C.cK_A, C.cM_A, C.kz.p, C.kp0.p, C.kp1.p, C.kq.p, C.red, false, C.stream, false);
+    hipEvent_t e0, e1;
+    TT_HIP(hipEventCreate(&e0));
+    TT_HIP(hipEventCreate(&e1));
+    TT_HIP(hipEventRecord(e0, C.stream));
+    for (int i = 0; i < nrep; i++)
+        tt::launch_fcg_apply(C.op_fine64, C.cK_A, C.cM_A, C.kz.p, C.kp0.p, C.kp1.p, C.kq.p, C.red, false, C.stream,
+                             false);
+    TT_HIP(hipEventRecord(e1, C.stream));
+    TT_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    TT_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / nrep;
+    TT_API_END
+}
+
 int tt_complete_initial_state(tt_ctx* ctx) {
     TT_API_BEGIN(ctx)
     C.complete_initial_state();
@@ -582,6 +615,15 @@ int tt_get_state(tt_ctx* ctx, int t, double* u, double* v, double* a, int on_dev
                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, C.stream));
     }
     C.sync();
+    TT_API_END
+}
+
+int tt_set_checkpoint(tt_ctx* ctx, int t, const double* u, const double* v, const double* a, int on_device) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(u && v && a, TT_ERR_ARG, "null state vector");
+    const long nd = C.g.nd;
+    DevIn ui(C, u, nd, on_device), vi(C, v, nd, on_device), ai(C, a, nd, on_device);
+    C.set_checkpoint(t, ui.p, vi.p, ai.p);
     TT_API_END
 }
 

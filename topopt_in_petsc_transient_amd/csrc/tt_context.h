@@ -4,6 +4,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,7 @@ struct MGLevel {
     Grid g;
     bool fp64 = true;                  // precision of this level's vectors and element matrices
     DBuf<uint8_t> mask, tbc4, tbc2;    // node mask; per-tile flags of clamped nodes (8x8x4, 8x8x2 tiles)
+    BatchTabStore btab;                // element batch tables of the tiles (full and edge tiles)
     DBuf<double> kf64, mf64;           // FP64 coefficients (coarse levels: averages of children)
     DBuf<float> kf32, mf32;            // mixed mode, finest level
     DBuf<bf16_t> kfbf, mfbf;           // mixed mode, coarse levels
@@ -95,6 +97,7 @@ struct Context {
     DBuf<double> rcnt;  // per node: sum of region indicators of the adjacent elements
     DBuf<double> F;     // load location vector (3 nn)
     DBuf<uint8_t> mask, tbc4, tbc2;
+    BatchTabStore btab;
     std::vector<double> Fhost;
     OpLevel op_fine64{};  // FP64 fine operator view (for FCG residuals)
     DBuf<double> KE64d, M864d, AF64d;
@@ -138,6 +141,9 @@ struct Context {
     // checkpointed runs: st_last = state T as the forward sweep left it; st_hi = the later of the two
     // states an adjoint step reads, kept from the previous (later) adjoint step
     DBuf<double> st_last, st_hi;
+    // steps whose states the caller gave (tt_set_checkpoint) in place of a forward sweep
+    std::set<int> given;
+    void set_checkpoint(int t, const double* u, const double* v, const double* a);
 
     // adjoint work
     DBuf<double> lam, lamT,

@@ -146,6 +146,21 @@ enum {
 // Elasticity operator on one level.  Coefficient storage CT is double, float or BF16.
 enum CoefKind { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2 };
 
+// Batch tables of the node tiles of one grid (see tt_apply_mfma.inc): per edge combination (bit 0: the
+// last tile in x, bit 1: in y, bit 2: in z) the offset and batch count of its table in `codes`.  The last
+// tile in a direction owns fewer nodes than a full tile when the grid's node count is not a multiple of
+// the tile extent; its table leaves out the elements that touch none of its nodes.
+struct BatchTab {
+    const int2* codes;
+    int off[8], nb[8];
+};
+struct BatchTabStore {
+    DBuf<int2> buf;
+    BatchTab t4{}, t2{};  // 8 x 8 x 4 and 8 x 8 x 2 node tiles
+};
+// tables of both tile depths for grid g, uploaded into st.buf
+void build_batch_tables(const Grid& g, BatchTabStore& st);
+
 struct OpLevel {
     Grid g;
     int coef_kind;         // CoefKind
@@ -162,6 +177,7 @@ struct OpLevel {
     float hs[2];
     const uint8_t* tbc4;   // per 8x8x4 / 8x8x2 tile: 1 if its staged node box holds a clamped node
     const uint8_t* tbc2;
+    BatchTab tab4, tab2;   // element batch tables of the 8x8x4 / 8x8x2 tiles
 };
 // Per-tile flags of 8 x 8 x tz node tiles: 1 if the tile's node box (one halo layer) holds a node
 // with a nonzero mask.
@@ -173,6 +189,10 @@ void tile_bc_flags(const Grid& g, const std::vector<uint8_t>& mask, int tz, std:
 long sym_packed_size(int n);
 void launch_sym_pack(const double* A, int n, double* P, hipStream_t s);
 void launch_sym_solve(const double* P, const double* b, double* x, int n, double* prow, double* pcol, hipStream_t s);
+// Dense N (cK K + cM M) N + I - N of a (coarsest) level from its FP64 element coefficients, on the device
+// (row-major n x n, n = g.nd; cleared first).
+void launch_dense_coarse(const Grid& g, const double* kf, const double* mf, double cK, double cM, const double* KE,
+                         const double* M8, const uint8_t* mask, double* A, hipStream_t s);
 // Column-major lower triangle of an n x n matrix copied into its upper triangle.
 void launch_mirror_lower(double* A, int n, hipStream_t s);
 constexpr int AF32_WORDS = 18 * 64 + 3 * 2 * 64 * 4;  // FP32 fragments followed by the FP16-split ones
@@ -203,9 +223,10 @@ void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, cons
 // diagonal of (cK K + cM M) with identity on clamped dofs, inverted
 void launch_diag_inv(const OpLevel& L, double cK, double cM, void* dinv, bool fp64, hipStream_t s);
 
-// FCG fused apply: p = z + beta*p_old (beta from scal), q = A p, partial p.q -> alpha = gamma / pq
+// FCG fused apply: p = z + beta*p_old (beta from scal), q = A p, partial p.q -> alpha = gamma / pq.
+// beta = false: the tile kernel alone, with the beta already in scal (measurement)
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
-                      double* q, Reducer& red, bool first, hipStream_t s);
+                      double* q, Reducer& red, bool first, hipStream_t s, bool beta = true);
 
 // vector kernels
 void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s);
@@ -332,6 +353,8 @@ void launch_mma_change(const double* x, double* xold, long n, Reducer& red, int 
 void launch_mma_kkt(const double* x, const double* df, const double* dg, const double* xmin, const double* xmax,
                     long n, double lam, Reducer& red, hipStream_t s);
 void launch_fill_scal(void* x, long n, double v, hipStream_t s);
+// x_i in [-1, 1) from a counter-based hash of (seed, i): deterministic, generated on the device
+void launch_fill_hash(void* x, long n, unsigned long long seed, bool fp64, hipStream_t s);
 // x = N.x (clamped dofs of the node mask set to 0)
 void launch_mask_dofs(double* x, const uint8_t* mask, long nd, hipStream_t s);
 void launch_scale(double* x, long n, double a, hipStream_t s);

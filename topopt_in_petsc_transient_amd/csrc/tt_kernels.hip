@@ -664,7 +664,7 @@ static int launch_tiles_ct(const OpLevel& L, const T* AF, int tx, int ty, int tz
     const int nb = 8 * ((ntiles + 7) / 8);
     hipLaunchKernelGGL((k_apply_tiles<T, CT, TZ_, MASKIN, Stage, Epi>), dim3(nb), dim3(PTile<TZ_>::NTH), 0, s,
                        gargs(L.g), tx, ty, ntiles, AF, (const CT*)L.kf, (const CT*)L.mf, cK, cM, L.mask,
-                       TZ_ == 4 ? L.tbc4 : L.tbc2, st, epi);
+                       TZ_ == 4 ? L.tbc4 : L.tbc2, TZ_ == 4 ? L.tab4 : L.tab2, st, epi);
     TT_HIP(hipGetLastError());
     return nb;
 }
@@ -848,11 +848,13 @@ constexpr int BSR = 1024;
 __global__ __launch_bounds__(BSR) void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal,
                                                   int first);
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
-                      double* q, Reducer& red, bool first, hipStream_t s) {
+                      double* q, Reducer& red, bool first, hipStream_t s, bool beta) {
     TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
-    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BSR), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
-                       first ? 1 : 0);
-    TT_HIP(hipGetLastError());
+    if (beta) {
+        hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BSR), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
+                           first ? 1 : 0);
+        TT_HIP(hipGetLastError());
+    }
     StageFcg<double> st{z, p_old, red.scal.p, first ? 1 : 0, 0.0};
     EpiFcg e{p_new, q, red.part(PB_PQ)};
     red.nblk[PB_PQ] = run_apply<double, false, true>(L, cK, cM, st, e, s);
@@ -913,6 +915,25 @@ void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s) {
     TT_HIP(hipGetLastError());
 }
 void launch_fill_scal(void* x, long n, double v, hipStream_t s) { launch_fill(x, v, n, true, s); }
+
+// x_i uniform in [-1, 1) from a counter-based hash of (seed, i) (splitmix64 finaliser): a deterministic
+// start vector made on the device, with no host loop or copy
+template <typename T>
+__global__ void k_fill_hash(T* __restrict__ x, long n, unsigned long long seed) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = (T)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+}
+void launch_fill_hash(void* x, long n, unsigned long long seed, bool fp64, hipStream_t s) {
+    if (n <= 0) return;
+    if (fp64) hipLaunchKernelGGL(k_fill_hash<double>, dim3(nblocks(n)), dim3(BS), 0, s, (double*)x, n, seed);
+    else hipLaunchKernelGGL(k_fill_hash<float>, dim3(nblocks(n)), dim3(BS), 0, s, (float*)x, n, seed);
+    TT_HIP(hipGetLastError());
+}
 
 // x = N.x: dof c of node n is zeroed where bit c of mask[n] is set
 __global__ void k_mask_dofs(double* __restrict__ x, const uint8_t* __restrict__ mask, long nd) {
@@ -1182,6 +1203,48 @@ void launch_mirror_lower(double* A, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_mirror_lower, dim3(nt, nt), dim3(256), 0, s, A, n);
     TT_HIP(hipGetLastError());
 }
+// ---- dense coarsest operator, assembled on the device: entry (i, j) of N (cK K + cM M) N + I - N, i = 3 p + c,
+// j = 3 q + d, for the 27 neighbours q of node p (all other entries stay 0 from the clear before).  One
+// thread per (i, neighbour, d) sums the element contributions in increasing element order, so the result
+// does not depend on scheduling (deterministic set-up).
+__global__ void k_dense_coarse(GridArgs g, const double* __restrict__ kf, const double* __restrict__ mf, double cK,
+                               double cM, const double* __restrict__ KE, const double* __restrict__ M8,
+                               const uint8_t* __restrict__ mask, int n, double* __restrict__ A) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)n * 81) return;
+    const int i = (int)(t / 81), r = (int)(t % 81);
+    const int nb = r / 3, d = r % 3;
+    const int p = i / 3, c = i % 3;
+    const int pi = p % g.nx, pj = (p / g.nx) % g.ny, pk = p / (g.nx * g.ny);
+    const int qi = pi + nb % 3 - 1, qj = pj + (nb / 3) % 3 - 1, qk = pk + nb / 9 - 1;
+    if (qi < 0 || qi >= g.nx || qj < 0 || qj >= g.ny || qk < 0 || qk >= g.nz) return;
+    const int q = qi + g.nx * (qj + g.ny * qk), j = 3 * q + d;
+    double val = 0.0;
+    if (((mask[p] >> c) & 1) || ((mask[q] >> d) & 1)) {
+        val = i == j ? 1.0 : 0.0;
+    } else {
+        // elements holding both nodes: origin (ei, ej, ek) with max(p, q) - 1 <= e <= min(p, q) per axis
+        for (int ek = max(max(pk, qk) - 1, 0); ek <= min(min(pk, qk), g.ez - 1); ek++)
+            for (int ej = max(max(pj, qj) - 1, 0); ej <= min(min(pj, qj), g.ey - 1); ej++)
+                for (int ei = max(max(pi, qi) - 1, 0); ei <= min(min(pi, qi), g.ex - 1); ei++) {
+                    const long e = ei + (long)g.ex * (ej + (long)g.ey * ek);
+                    const int la = corner_id(pi - ei, pj - ej, pk - ek), lb = corner_id(qi - ei, qj - ej, qk - ek);
+                    double v = cK * kf[e] * KE[(3 * la + c) * 24 + 3 * lb + d];
+                    if (c == d) v += cM * mf[e] * M8[la * 8 + lb];
+                    val += v;
+                }
+    }
+    A[(size_t)i * n + j] = val;
+}
+void launch_dense_coarse(const Grid& g, const double* kf, const double* mf, double cK, double cM, const double* KE,
+                         const double* M8, const uint8_t* mask, double* A, hipStream_t s) {
+    const int n = (int)g.nd;
+    TT_HIP(hipMemsetAsync(A, 0, (size_t)n * n * sizeof(double), s));
+    hipLaunchKernelGGL(k_dense_coarse, dim3(nblocks((long)n * 81)), dim3(BS), 0, s, gargs(g), kf, mf, cK, cM, KE, M8,
+                       mask, n, A);
+    TT_HIP(hipGetLastError());
+}
+
 // ---- symmetric coarsest solve on the packed lower triangle: 64 x 64 tiles (I, J), I >= J, stored
 // contiguously in row-major order, tile index I (I + 1) / 2 + J.  Half the bytes of the full matrix.
 constexpr int ST = 64;

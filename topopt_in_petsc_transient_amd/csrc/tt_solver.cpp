@@ -111,6 +111,9 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
         lev->op.mask = lev->mask.p;
         lev->op.tbc4 = lev->tbc4.p;
         lev->op.tbc2 = lev->tbc2.p;
+        build_batch_tables(g, lev->btab);
+        lev->op.tab4 = lev->btab.t4;
+        lev->op.tab2 = lev->btab.t2;
         lev->op.KE64 = lev->KE64.p;
         lev->op.M864 = lev->M864.p;
         lev->op.KE32 = lev->KE32.p;
@@ -122,7 +125,9 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
     }
     const Grid& gc = L.back()->g;
     nc = (int)gc.nd;
-    dense = (long)nc * nc * 8 <= (320l << 20);  // explicit FP64 inverse up to 320 MB
+    // explicit FP64 inverse up to 320 MB (6,476 dofs), or up to the caller's limit
+    const long dmax = c->prm.coarse_direct_max > 0 ? c->prm.coarse_direct_max : 6476;
+    dense = nc <= dmax && (long)nc * nc * 8 <= (1l << 34);
     cb.alloc(nc);
     cx.alloc(nc);
     if (dense) {
@@ -133,7 +138,7 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
             sprow.alloc(ntiles * 64);
             spcol.alloc(ntiles * 64);
         }
-        dinfo.alloc(1);
+        dinfo.alloc(2);
         rocblas_handle h = nullptr;
         TT_CHECK(rocblas_create_handle(&h) == rocblas_status_success, TT_ERR_HIP, "rocblas_create_handle failed");
         blas = h;
@@ -186,20 +191,7 @@ double Multigrid::power_lmax(int l, int iters) {
     MGLevel& lv = *L[l];
     hipStream_t s = ctx->stream;
     const long nd = lv.g.nd;
-    std::vector<double> h(nd);
-    uint64_t st = 0x9E3779B97F4A7C15ull;
-    for (long i = 0; i < nd; i++) {
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        h[i] = ((double)(st >> 11) / 9007199254740992.0) * 2.0 - 1.0;
-    }
-    if (lv.fp64) {
-        TT_HIP(hipMemcpyAsync(lv.x.p, h.data(), nd * 8, hipMemcpyHostToDevice, s));
-    } else {
-        std::vector<float> hf(h.begin(), h.end());
-        TT_HIP(hipMemcpyAsync(lv.x.p, hf.data(), nd * 4, hipMemcpyHostToDevice, s));
-        TT_HIP(hipStreamSynchronize(s));
-    }
-    TT_HIP(hipStreamSynchronize(s));
+    launch_fill_hash(lv.x.p, nd, 0x9E3779B97F4A7C15ull, lv.fp64, s);  // deterministic start vector
     for (int it = 0; it < iters; it++) {
         launch_apply_bc(lv.op, cK, cM, lv.x.p, lv.r.p, lv.fp64, s);
         launch_norm_power(lv.r.p, lv.dinv.p, lv.x.p, nd, ctx->red, lv.fp64, s);
@@ -215,64 +207,31 @@ Multigrid::~Multigrid() {
 void Multigrid::assemble_coarse_inverse() {
     MGLevel& lc = *L.back();
     const Grid& g = lc.g;
-    std::vector<double> kf(g.nel), mf(g.nel);
     const double* kd = L.size() > 1 ? lc.kf64.p : (const double*)lc.op.kf;
     const double* md = L.size() > 1 ? lc.mf64.p : (const double*)lc.op.mf;
     if (L.size() == 1 && mixed) {  // single-level mixed run: use the context's FP64 coefficients
         kd = ctx->kf.p;
         md = ctx->mf.p;
     }
-    TT_HIP(hipMemcpyAsync(kf.data(), kd, g.nel * 8, hipMemcpyDeviceToHost, ctx->stream));
-    TT_HIP(hipMemcpyAsync(mf.data(), md, g.nel * 8, hipMemcpyDeviceToHost, ctx->stream));
-    TT_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<uint8_t> m;
-    build_mask(g, ctx->prm.boundaries, m);
-    double ke[576], m8[64];
-    hex8_stiffness(g.hx, g.hy, g.hz, ctx->prm.nu, ke);
-    hex8_mass(g.hx, g.hy, g.hz, m8);
     const int n = nc;
-    std::vector<double> A((size_t)n * n, 0.0);
-    for (long e = 0; e < g.nel; e++) {
-        const long i = e % g.ex, j = (e / g.ex) % g.ey, k = e / ((long)g.ex * g.ey);
-        const long n0 = i + (long)g.nx * (j + (long)g.ny * k), sy = g.nx, sz = (long)g.nx * g.ny;
-        const long nodes[8] = {n0, n0 + 1, n0 + 1 + sy, n0 + sy, n0 + sz, n0 + 1 + sz, n0 + 1 + sy + sz, n0 + sy + sz};
-        const double ck = cK * kf[e], cm = cM * mf[e];
-        for (int p = 0; p < 24; p++)
-            for (int q = 0; q < 24; q++) {
-                const int a = p / 3, b = q / 3;
-                double v = ck * ke[24 * p + q] + ((p % 3) == (q % 3) ? cm * m8[8 * a + b] : 0.0);
-                A[(size_t)(3 * nodes[a] + p % 3) * n + (3 * nodes[b] + q % 3)] += v;
-            }
-    }
-    for (long nn = 0; nn < g.nn; nn++)
-        for (int c = 0; c < 3; c++)
-            if ((m[nn] >> c) & 1) {
-                const long d = 3 * nn + c;
-                for (int q = 0; q < n; q++) {
-                    A[(size_t)d * n + q] = 0.0;
-                    A[(size_t)q * n + d] = 0.0;
-                }
-                A[(size_t)d * n + d] = 1.0;
-            }
+    // the dense coarsest operator, assembled on the device from the level's FP64 coefficients
+    launch_dense_coarse(g, kd, md, cK, cM, lc.KE64.p, lc.M864.p, lc.mask.p, Ainv.p, ctx->stream);
     // A^-1 on the device: Cholesky (potrf) then the inverse from the factor (potri), lower
     // triangle in column-major order, mirrored into the upper triangle for the GEMV.
     hipStream_t s = ctx->stream;
-    TT_HIP(hipMemcpyAsync(Ainv.p, A.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, s));
     rocblas_handle h = (rocblas_handle)blas;
+    // (both status words are read back once, at the end: one synchronisation for the whole set-up)
     TT_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p) == rocblas_status_success, TT_ERR_HIP,
              "rocsolver_dpotrf failed");
-    int info = 0;
-    TT_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    TT_HIP(hipStreamSynchronize(s));
-    TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
-    TT_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p) == rocblas_status_success, TT_ERR_HIP,
+    TT_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p + 1) == rocblas_status_success, TT_ERR_HIP,
              "rocsolver_dpotri failed");
-    TT_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    TT_HIP(hipStreamSynchronize(s));
-    TT_CHECK(info == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
     launch_mirror_lower(Ainv.p, n, s);
     launch_sym_pack(Ainv.p, n, AinvP.p, s);
+    int info[2] = {0, 0};
+    TT_HIP(hipMemcpyAsync(info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, s));
     TT_HIP(hipStreamSynchronize(s));
+    TT_CHECK(info[0] == 0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
+    TT_CHECK(info[1] == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
 }
 
 std::vector<char**> Multigrid::rotating() {
@@ -301,14 +260,7 @@ void Multigrid::setup_operator(double cK_, double cM_) {
         // Chebyshev coarse solve in FP64 on [clmax/30, 1.05 clmax] of D^-1 A
         OpLevel op = coarse_op64();
         launch_diag_inv(op, cK, cM, cdinv.p, true, s);
-        std::vector<double> h(nc);
-        uint64_t st = 0x2545F4914F6CDD1Dull;
-        for (int i = 0; i < nc; i++) {
-            st = st * 6364136223846793005ull + 1442695040888963407ull;
-            h[i] = ((double)(st >> 11) / 9007199254740992.0) * 2.0 - 1.0;
-        }
-        TT_HIP(hipMemcpyAsync(cx.p, h.data(), nc * 8, hipMemcpyHostToDevice, s));
-        TT_HIP(hipStreamSynchronize(s));
+        launch_fill_hash(cx.p, nc, 0x2545F4914F6CDD1Dull, true, s);
         for (int it = 0; it < 20; it++) {
             launch_apply_bc(op, cK, cM, cx.p, cd.p, true, s);
             launch_norm_power(cd.p, cdinv.p, cx.p, nc, ctx->red, true, s);

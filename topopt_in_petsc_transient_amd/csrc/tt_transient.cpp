@@ -38,6 +38,7 @@ double Context::load_intensity(int step) const {
 // ------------------------------------------------------------------ forward
 void Context::complete_initial_state() {
     TT_CHECK(density_set, TT_ERR_STATE, "tt_set_density must be called first");
+    given.clear();
     if (reduced) {
         reduced_initial_state();
         initial_done = true;
@@ -228,7 +229,7 @@ const double* Context::state_at(int t) {
             kb = (int)i;
         }
     }
-    if (!store_all && t == prm.nsteps) return st_last.p;
+    if (!store_all && t == prm.nsteps && given.empty()) return st_last.p;
     TT_CHECK(kb >= 0, TT_ERR_INTERNAL, "no stored state below the requested one");
     const double* src = slots[kb].p;
     if (scratch_step > base && scratch_step <= t) {
@@ -245,8 +246,44 @@ const double* Context::state_at(int t) {
     return st_cur.p;
 }
 
+void Context::set_checkpoint(int t, const double* u, const double* v, const double* ac) {
+    TT_CHECK(!reduced, TT_ERR_STATE, "no checkpoint states with the reduced-order model");
+    TT_CHECK(initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
+    TT_CHECK(t >= 0 && t <= prm.nsteps, TT_ERR_ARG, "time step out of range");
+    const long nd = g.nd;
+    if (given.empty()) {  // the first given state replaces the forward sweep's record; state 0 stays
+        for (size_t i = 1; i < slot_step.size(); i++) slot_step[i] = -1;
+        slot_step[0] = 0;
+        given.insert(0);
+        scratch_step = -1;
+    }
+    int k = -1;
+    if (t == 0) {
+        k = 0;
+    } else if (store_all) {
+        k = t;
+    } else {
+        for (size_t i = 1; i < slot_step.size() && k < 0; i++)
+            if (slot_step[i] == t) k = (int)i;
+        for (size_t i = 1; i < slot_step.size() && k < 0; i++)
+            if (slot_step[i] < 0) k = (int)i;
+        TT_CHECK(k > 0, TT_ERR_ARG, "more checkpoint states than checkpoint slots");
+    }
+    double* s = slots[k].p;
+    launch_copy(u, s, nd, true, true, stream);
+    launch_copy(v, s + nd, nd, true, true, stream);
+    launch_copy(ac, s + 2 * nd, nd, true, true, stream);
+    // no acceleration history comes with a file: the next solve's guess extrapolates a constant acceleration
+    for (int h = 3; h < kStateVecs; h++) launch_copy(ac, s + (size_t)h * nd, nd, true, true, stream);
+    slot_step[k] = t;
+    given.insert(t);
+    forward_done = true;
+    sync();
+}
+
 void Context::forward() {
     TT_CHECK(initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
+    given.clear();
     scratch_step = -1;
     if (reduced) {
         reduced_forward();
@@ -400,7 +437,61 @@ void Context::objective_gradient(double* fx_host, double* dfdx_dev) {
         reduced_adjoint(grad_state, sens);
     } else {
         launch_copy(lamT.p, lam.p, nd, true, true, stream);
-        if (store_all) {
+        if (!given.empty()) {
+            // States given by the caller (the reference's checkpoint files, read back as
+            // ComputeObjectiveSensitivities does, LinearElasticity.cc:1651-1680): the blocks between
+            // consecutive given steps are recomputed from their first state (SolveStateBlock) and the
+            // adjoint runs over each block, from the last block back.
+            std::vector<int> cps(given.begin(), given.end());
+            Revolve rv;
+            rv.c = this;
+            if (store_all) {
+                for (size_t b = 0; b < cps.size(); b++) {
+                    const int l = cps[b], r = b + 1 < cps.size() ? cps[b + 1] : T;
+                    for (int t = l + 1; t <= r; t++) {
+                        if (slot_step[t] == t) continue;  // a given state is kept as given
+                        newmark_step(slots[t - 1].p, slots[t].p, t);
+                        slot_step[t] = t;
+                    }
+                }
+                for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
+            } else {
+                const double* later = nullptr;
+                int later_step = -1;
+                rv.adjoint_at = [&](int t, const double* prev) {
+                    adjoint_step(t, later, prev);
+                    if (prev == st_cur.p) {
+                        std::swap(st_hi.p, st_cur.p);
+                        later = st_hi.p;
+                        later_step = -1;
+                    } else {
+                        later = prev;
+                        later_step = t - 1;
+                    }
+                };
+                rv.on_release = [&](int step) {
+                    if (step != later_step) return;
+                    launch_copy(later, st_hi.p, 3 * nd, true, true, stream);
+                    later = st_hi.p;
+                    later_step = -1;
+                };
+                int nfree = 0;
+                for (size_t i = 1; i < slot_step.size(); i++) nfree += slot_step[i] < 0 ? 1 : 0;
+                for (int b = (int)cps.size() - 1; b >= 0; b--) {
+                    const int l = cps[b], r = b + 1 < (int)cps.size() ? cps[b + 1] : T;
+                    if (r == l) continue;
+                    if (b + 1 == (int)cps.size()) {  // state T: the last block recomputed once
+                        const double* sT = rv.advance(l, T, nullptr);
+                        launch_copy(sT, st_last.p, 3 * nd, true, true, stream);
+                        later = st_last.p;
+                    } else {
+                        later = rv.slot_of(r);  // a given state, never released
+                    }
+                    later_step = -1;
+                    rv.reverse(l, r, nfree);
+                }
+            }
+        } else if (store_all) {
             for (int t = T; t >= 1; t--) adjoint_step(t, slots[t].p, slots[t - 1].p);
         } else {
             // The adjoint step at t reads states t and t-1 only.  The sweep runs t = T, T-1, ..., 1, so
