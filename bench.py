@@ -127,6 +127,10 @@ def forward_run(capi, x, precision, nsteps):
     u, _, _ = ctx.get_state(nsteps)
     res = {"seconds": el, "iters_per_solve": st.krylov_iters / max(1, st.solves), "solves": st.solves,
            "max_relres": st.max_relres, "lambda_max": list(st.lambda_max)[:st.nlvls]}
+    # the multigrid preconditioner of the last Newmark operator alone: GPU time per application, and the
+    # host's time to enqueue one (close to the GPU time: the V-cycle is bound by kernel launches)
+    g_ms, h_ms = ctx.time_vcycle(100)
+    res["vcycle_ms"] = {"gpu": g_ms, "host_enqueue": h_ms}
     ctx.close()
     return res, u
 

@@ -241,6 +241,13 @@ int tt_time_op_apply(tt_ctx* ctx, int which, int prec, const void* u, void* y, i
    context's stream; average milliseconds per launch. */
 int tt_time_fcg_apply(tt_ctx* ctx, int nrep, double* ms_per_apply);
 
+/* Measurement, no counterpart in the reference: nrep multigrid preconditioner applications z = B r of
+   the Krylov iteration (the V-cycle of the last operator set up, tt_complete_initial_state or a solve
+   before), back to back on the context's stream.  ms_gpu: average milliseconds per application between
+   HIP events; ms_host: average host milliseconds to enqueue one (when it is close to ms_gpu, the
+   V-cycle is bound by kernel launches, not by the GPU). */
+int tt_time_vcycle(tt_ctx* ctx, int nrep, double* ms_gpu, double* ms_host);
+
 /* No counterpart in the reference: the allocation-free form of tt_volume_constraint
    (LinearElasticity.cc:1283-1296), for callers that cannot allocate device memory (code under HIP graph
    capture, tight serving loops).  No context: xphys (nel doubles), gx (one double) and dgdx (nel doubles,

@@ -29,7 +29,7 @@ EXPORTED = [
     "tt_num_nodes", "tt_num_elements", "tt_time_op_apply", "tt_set_material", "tt_set_filter_radius",
     "tt_mma_get_state", "tt_mma_set_state", "tt_get_restart_vectors", "tt_set_restart_vectors",
     "tt_petsc_write_vecs", "tt_petsc_read_vecs", "tt_get_reduced_model", "tt_set_initial_displacement",
-    "tt_volume_constraint_noalloc", "tt_time_fcg_apply", "tt_set_checkpoint",
+    "tt_volume_constraint_noalloc", "tt_time_fcg_apply", "tt_set_checkpoint", "tt_time_vcycle",
 ]
 
 
@@ -110,6 +110,7 @@ def lib():
             "tt_set_initial_displacement": [vp, vp, ip],
             "tt_volume_constraint_noalloc": [vp, ctypes.c_long, ctypes.c_double, vp, vp, vp],
             "tt_time_fcg_apply": [vp, ip, dp],
+            "tt_time_vcycle": [vp, ip, dp, dp],
             "tt_set_checkpoint": [vp, ip, vp, vp, vp, ip],
         }
         for name, args in sig.items():
@@ -212,6 +213,12 @@ class Context:
         ms = ctypes.c_double()
         self._chk(lib().tt_time_fcg_apply(self.h, nrep, ctypes.byref(ms)))
         return ms.value
+
+    def time_vcycle(self, nrep):
+        """(GPU ms, host enqueue ms) per multigrid preconditioner application."""
+        g, h = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().tt_time_vcycle(self.h, nrep, ctypes.byref(g), ctypes.byref(h)))
+        return g.value, h.value
 
     def complete_initial_state(self):
         self._chk(lib().tt_complete_initial_state(self.h))

@@ -1,4 +1,5 @@
 // Context set-up, PDE filter, MMA driver and the extern "C" entry points.
+#include <chrono>
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -36,6 +37,7 @@ Context::Context(const tt_params& p) : prm(p) {
     check_params(p);
     TT_HIP(hipGetDevice(&device));
     TT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    TT_HIP(hipEventCreateWithFlags(&look_ev, hipEventDisableTiming));
     g = make_grid(p.nelx, p.nely, p.nelz, p.xc[1] - p.xc[0], p.xc[3] - p.xc[2], p.xc[5] - p.xc[4]);
     const long nd = g.nd, nn = g.nn, nel = g.nel;
     const long apply_blocks = 8 * (((long)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 1) / 2) + 7) / 8);
@@ -220,6 +222,7 @@ void Context::set_filter_radius(double rmin) {
 Context::~Context() {
     clear_graphs();
     if (projhost) (void)hipHostFree(projhost);
+    if (look_ev) (void)hipEventDestroy(look_ev);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -588,6 +591,33 @@ int tt_time_fcg_apply(tt_ctx* ctx, int nrep, double* ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = (double)t / nrep;
+    TT_API_END
+}
+
+int tt_time_vcycle(tt_ctx* ctx, int nrep, double* ms_gpu, double* ms_host) {
+    TT_API_BEGIN(ctx)
+    TT_CHECK(C.initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
+    TT_CHECK(ms_gpu && ms_host && nrep > 0, TT_ERR_ARG, "bad tt_time_vcycle arguments");
+    const long nd = C.g.nd;
+    tt::launch_fill_hash(C.kr.p, nd, 0x3C6EF372FE94F82Bull, true, C.stream);
+    tt::launch_fill_hash(C.krp.p, nd, 0x6A09E667F3BCC908ull, true, C.stream);
+    C.mg.apply_fcg(C.kr.p, C.krp.p, C.kz.p, true);
+    C.sync();
+    hipEvent_t e0, e1;
+    TT_HIP(hipEventCreate(&e0));
+    TT_HIP(hipEventCreate(&e1));
+    TT_HIP(hipEventRecord(e0, C.stream));
+    const auto h0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < nrep; i++) C.mg.apply_fcg(C.kr.p, C.krp.p, C.kz.p, i == 0);
+    const auto h1 = std::chrono::steady_clock::now();
+    TT_HIP(hipEventRecord(e1, C.stream));
+    TT_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    TT_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_gpu = (double)t / nrep;
+    *ms_host = std::chrono::duration<double, std::milli>(h1 - h0).count() / nrep;
     TT_API_END
 }
 

@@ -191,6 +191,10 @@ struct Context {
     SolveResult fcg(double cK, double cM, bool use_mg, SolveKind kind, const double* b, double* x);
     int fcg_hist[kSolveKinds][kHintLen] = {};  // iteration counts of the last solves of each kind (0: none)
     void sync();
+    // waits for the stream by polling an event: the Krylov loop's residual looks, where the GPU idles
+    // until the host has seen the result and queued the next launches
+    hipEvent_t look_ev = nullptr;
+    void wait_look();
     double read_scal(int slot);
     double reduce1(int region);  // host sum of a partial region (synchronises)
 

@@ -146,16 +146,18 @@ enum {
 // Elasticity operator on one level.  Coefficient storage CT is double, float or BF16.
 enum CoefKind { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2 };
 
-// Batch tables of the node tiles of one grid (see tt_apply_mfma.inc): per edge combination (bit 0: the
-// last tile in x, bit 1: in y, bit 2: in z) the offset and batch count of its table in `codes`.  The last
-// tile in a direction owns fewer nodes than a full tile when the grid's node count is not a multiple of
-// the tile extent; its table leaves out the elements that touch none of its nodes.
+// Batch tables of the node tiles of one grid (see tt_apply_mfma.inc): per tile position (bits 0/1: the
+// first / last tile in x, bits 2/3 in y, bits 4/5 in z) the offset and batch count of its table in
+// `codes`.  A tile's table holds the elements that exist and touch one of its nodes: the first tile in
+// a direction has no elements on its low side, and the last tile owns fewer nodes than a full tile
+// when the grid's node count is not a multiple of the tile extent.
 struct BatchTab {
-    const int2* codes;
-    int off[8], nb[8];
+    const int4* codes;
+    const int2* at;  // per position: (offset, batch count), in device memory (keeps the launch arguments small)
 };
 struct BatchTabStore {
-    DBuf<int2> buf;
+    DBuf<int4> buf;
+    DBuf<int2> at;
     BatchTab t4{}, t2{};  // 8 x 8 x 4 and 8 x 8 x 2 node tiles
 };
 // tables of both tile depths for grid g, uploaded into st.buf

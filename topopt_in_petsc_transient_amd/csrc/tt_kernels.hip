@@ -339,20 +339,29 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
 
 #include "tt_apply_mfma.inc"
 
-// ---- epilogues.  put() handles one owned node (inside the grid) and may add to the thread's
-// reduction share; done() is called once by every thread of the block at the end (reducing
-// epilogues write their block partial there).  operator() is the one-node-per-thread form.
+// ---- epilogues.  fetch() loads the per-node operands an epilogue reads (its Pre) for one owned
+// node; the tile kernels issue it together with the staging loads, so that those loads are in
+// flight while the element contraction runs.  put() then handles that node (inside the grid) and
+// may add to the thread's reduction share; done() is called once by every thread of the block at
+// the end (reducing epilogues write their block partial there).  operator() is the
+// one-node-per-thread form.
 #define TT_EPI_CALL                                                                                      \
     template <class S, typename V>                                                                     \
     __device__ void operator()(bool in, int n, uint8_t cl, const V (&yK)[3], const V (&yM)[3],          \
                                const S& st) const {                                                     \
         double acc[2] = {0.0, 0.0};                                                                    \
         if (in) {                                                                                      \
+            Pre pre;                                                                                   \
+            fetch(n, st, pre);                                                                         \
             const V uo[3] = {st.k(3 * n), st.k(3 * n + 1), st.k(3 * n + 2)};                           \
-            put(n, cl, yK, yM, st, uo, acc);                                                           \
+            put(n, cl, yK, yM, st, uo, acc, pre);                                                      \
         }                                                                                              \
         done(acc);                                                                                     \
     }
+#define TT_NO_PRE                                                     \
+    struct Pre {};                                                    \
+    template <class S>                                                \
+    __device__ __forceinline__ void fetch(int, const S&, Pre&) const {}
 
 // Input value of dof i for an epilogue: the staged value uo (free dofs), re-read through the stage
 // only on clamped dofs, where the staged copy is masked.
@@ -366,8 +375,10 @@ __device__ __forceinline__ T input_at(const S& st, uint8_t cl, int c, int i, con
 template <typename T>
 struct EpiBC {  // y = A u on free dofs, u on clamped
     T* y;
+    TT_NO_PRE
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&)[3], double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&)[3],
+                        double (&)[2], const Pre&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             T v = yK[c] + yM[c];
@@ -383,8 +394,10 @@ template <typename T>
 struct EpiBCDot {  // y = A u (BC form) and partials of u.y
     T* y;
     double* part;
+    TT_NO_PRE
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3], double (&acc)[2]) const {
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
+                        double (&acc)[2], const Pre&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const T u = input_at(st, cl, c, 3 * n + c, uo);
@@ -404,13 +417,22 @@ template <typename T>
 struct EpiResid {  // r = b - A x
     const T* b;
     T* r;
+    struct Pre {
+        T b[3];
+    };
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&)[3], double (&)[2]) const {
+    __device__ __forceinline__ void fetch(int n, const S&, Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) p.b[c] = b[3 * n + c];
+    }
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&)[3],
+                        double (&)[2], const Pre& p) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             T ax = yK[c] + yM[c];
             if ((cl >> c) & 1) ax = st.k(3 * n + c);
-            r[3 * n + c] = b[3 * n + c] - ax;
+            r[3 * n + c] = p.b[c] - ax;
         }
     }
     __device__ void done(double (&)[2]) const {}
@@ -423,13 +445,22 @@ struct EpiResidDots {
     double* r;
     double* pbb;
     double* prr;
+    struct Pre {
+        double b[3];
+    };
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st, const double (&)[3], double (&acc)[2]) const {
+    __device__ __forceinline__ void fetch(int n, const S&, Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) p.b[c] = b[3 * n + c];
+    }
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st,
+                        const double (&)[3], double (&acc)[2], const Pre& p) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             double ax = yK[c] + yM[c];
             if ((cl >> c) & 1) ax = st.k(3 * n + c);
-            const double bi = b[3 * n + c], ri = bi - ax;
+            const double bi = p.b[c], ri = bi - ax;
             r[3 * n + c] = ri;
             acc[0] = fma(bi, bi, acc[0]);
             acc[1] = fma(ri, ri, acc[1]);
@@ -455,17 +486,30 @@ struct EpiCheb {  // r = b - A x; dn = c0 d + c1 dinv r; xn = x + dn
     T c0, c1;
     T* xn;
     T* dn;
+    struct Pre {
+        T b[3], d[3], di[3];
+    };
+    template <class S>
+    __device__ __forceinline__ void fetch(int n, const S&, Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            p.b[c] = b[i];
+            p.di[c] = dinv[i];
+            p.d[c] = d ? d[i] : (T)0;
+        }
+    }
     template <class S>
     __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
-                        double (&)[2]) const {
+                        double (&)[2], const Pre& p) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
             const T x = input_at(st, cl, c, i, uo);
             const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
-            const T r = b[i] - ax;
-            T dd = c1 * dinv[i] * r;
-            if (d) dd = fma(c0, d[i], dd);
+            const T r = p.b[c] - ax;
+            T dd = c1 * p.di[c] * r;
+            if (d) dd = fma(c0, p.d[c], dd);
             xn[i] = x + dd;
             if (dn) dn[i] = dd;
         }
@@ -483,17 +527,29 @@ struct EpiChebFirst {
     T c0, c1;
     T* xn;
     T* dn;
+    struct Pre {
+        T b[3], di[3];
+    };
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
-                        double (&)[2]) const {
+    __device__ __forceinline__ void fetch(int n, const S& st, Pre& p) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
-            const T bi = st.bval(i);
+            p.b[c] = st.bval(i);
+            p.di[c] = dinv[i];
+        }
+    }
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
+                        double (&)[2], const Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            const T bi = p.b[c];
             if (FROM64) bout[i] = bi;
             const T x = input_at(st, cl, c, i, uo);
             const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
-            const T dd = fma(c0, x, c1 * dinv[i] * (bi - ax));
+            const T dd = fma(c0, x, c1 * p.di[c] * (bi - ax));
             xn[i] = x + dd;
             dn[i] = dd;
         }
@@ -515,19 +571,36 @@ struct EpiChebOut {
     const double* rp;
     int first;
     double* part;
+    struct Pre {
+        T b[3], d[3], di[3];
+        double r[3], rp[3];
+    };
     template <class S>
-    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3], double (&acc)[2]) const {
+    __device__ __forceinline__ void fetch(int n, const S&, Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            p.b[c] = b[i];
+            p.di[c] = dinv[i];
+            p.d[c] = d ? d[i] : (T)0;
+            p.r[c] = r[i];
+            p.rp[c] = first ? 0.0 : rp[i];
+        }
+    }
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
+                        double (&acc)[2], const Pre& p) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
             const T x = input_at(st, cl, c, i, uo);
             const T ax = ((cl >> c) & 1) ? x : yK[c] + yM[c];
-            T dd = c1 * dinv[i] * (b[i] - ax);
-            if (d) dd = fma(c0, d[i], dd);
+            T dd = c1 * p.di[c] * (p.b[c] - ax);
+            if (d) dd = fma(c0, p.d[c], dd);
             const double zi = (double)(x + dd);
             z[i] = zi;
-            acc[0] = fma(zi, r[i], acc[0]);
-            if (!first) acc[1] = fma(zi, rp[i], acc[1]);
+            acc[0] = fma(zi, p.r[c], acc[0]);
+            if (!first) acc[1] = fma(zi, p.rp[c], acc[1]);
         }
     }
     __device__ void done(double (&acc)[2]) const { write_partials<2>(acc, part); }
@@ -537,8 +610,10 @@ struct EpiChebOut {
 struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
     double* yk;
     double* ym;
+    TT_NO_PRE
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&,
+                        const double (&)[3], double (&)[2], const Pre&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const bool clamped = (cl >> c) & 1;
@@ -552,8 +627,10 @@ struct EpiSplit {  // yk = N.K(N u), ym = N.M(N u)
 
 struct EpiRaw {  // y = yK + yM, no BC
     double* y;
+    TT_NO_PRE
     template <class S>
-    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
+    __device__ void put(int n, uint8_t, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3],
+                        double (&)[2], const Pre&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) y[3 * n + c] = yK[c] + yM[c];
     }
@@ -565,12 +642,21 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     const double* F;
     double fs;
     double* R;
+    struct Pre {
+        double F[3];
+    };
     template <class S>
-    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3], double (&)[2]) const {
+    __device__ __forceinline__ void fetch(int n, const S&, Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) p.F[c] = F[3 * n + c];
+    }
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S&, const double (&)[3],
+                        double (&)[2], const Pre& p) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
-            double r = fs * F[i];
+            double r = fs * p.F[c];
             r += yK[c];
             r += yM[c];
             R[i] = ((cl >> c) & 1) ? 0.0 : r;
@@ -584,9 +670,10 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
     double* pnew;
     double* q;
     double* part;
+    TT_NO_PRE
     template <class S>
     __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st,
-                        const double (&uo)[3], double (&acc)[2]) const {
+                        const double (&uo)[3], double (&acc)[2], const Pre&) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
@@ -604,6 +691,7 @@ struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
     TT_EPI_CALL
 };
 #undef TT_EPI_CALL
+#undef TT_NO_PRE
 
 static inline dim3 apply_grid(const Grid& g) {
     return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + TZ - 1) / TZ);

@@ -419,6 +419,15 @@ void Multigrid::apply(const double* r, double* z) {
 // ------------------------------------------------------------------ context-level Krylov
 void Context::sync() { TT_HIP(hipStreamSynchronize(stream)); }
 
+void Context::wait_look() {
+    TT_HIP(hipEventRecord(look_ev, stream));
+    for (;;) {
+        const hipError_t e = hipEventQuery(look_ev);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) TT_HIP(e);
+    }
+}
+
 double Context::read_scal(int slot) {
     TT_HIP(hipMemcpyAsync(red.host + slot, red.scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, stream));
     TT_HIP(hipStreamSynchronize(stream));
@@ -533,7 +542,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
         // r.r partials, the device's count of iterations that moved x and b.b, as the kernels stored
         // them into mapped host memory: no copy, just the wait
         double rr = 0.0;
-        sync();
+        wait_look();
         {
             const volatile double* lk = red.look;
             const int nb = red.nblk[PB_RR];
