@@ -34,14 +34,15 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 PEAK_FP64 = 78.6e12        # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_FP32 = 157.3e12       # MI355X dense FP32 matrix, flop/s
 PEAK_FP16 = 2.5e15         # MI355X dense FP16 matrix, flop/s
-# matrix instructions of one FP32 A apply at 128x64x64: 17 x 9 x 17 tiles of 8x8x4 nodes, 28 batches of
-# 16 elements per tile, 9 FP16 16x16x32 instructions (16,384 flops each) per batch
-MFMA_EXEC_A32 = 17 * 9 * 17 * 28 * 9 * 16384
-# HBM bytes per launch of the FP32 A apply at 128x64x64 (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc,
-# scaled by the counters' reading for a copy kernel of known size); None until measured.
-TRAFFIC_A32 = 20751964  # FETCH 6936.6 KiB x 1.99922 + WRITE 6397.8 KiB x 1.0 (copy-calibrated)
-# the same for the FP64 flexible-CG apply; None until measured
-TRAFFIC_FCG64 = None
+# matrix instructions of one FP32 A apply at 128x64x64, as rocprofv3 counts them (SQ_INSTS_MFMA summed
+# over the dispatch): 520,058 v_mfma_f32_16x16x32_f16 of 16,384 flops each (2,601 tiles of 8x8x4 nodes;
+# the tables per tile position leave out elements outside the grid and edge-tile elements that touch
+# none of the tile's nodes)
+MFMA_EXEC_A32 = 520058 * 16384
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, FETCH scaled by 1.99922, the factor the
+# counters read for a device copy of 64 MiB; WRITE by 1.0)
+TRAFFIC_A32 = 20985242     # FP32 A apply: FETCH 7050.6 KiB, WRITE 6397.7 KiB
+TRAFFIC_FCG64 = 74640599   # FP64 flexible-CG apply: FETCH 23576.2 KiB, WRITE 25757.2 KiB
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
@@ -366,8 +367,8 @@ def main():
                   "128x64x64, 8x8x4-node tiles, FP16-split v_mfma_f32_16x16x32_f16)",
         "kernel_ms": a32["fp32_ms"], "flops_per_launch": fl,
         # the FP32-accurate contraction runs as three FP16 products on the matrix cores: executed
-        # matrix-instruction work per launch (2,601 tiles x 28 batches x 9 v_mfma_f32_16x16x32_f16
-        # of 16,384 flops) against the FP16 dense peak
+        # matrix-instruction work per launch (the counted v_mfma_f32_16x16x32_f16, 16,384 flops each)
+        # against the FP16 dense peak
         "matrix_pipe": {"instr": "v_mfma_f32_16x16x32_f16", "executed_flops_per_launch": MFMA_EXEC_A32,
                         "achieved": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12,
                         "unit": "TFLOP/s", "frac": MFMA_EXEC_A32 / (a32["fp32_ms"] * 1e-3) / PEAK_FP16}}
