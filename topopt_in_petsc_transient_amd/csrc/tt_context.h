@@ -152,7 +152,8 @@ struct Context {
 
     // PDE filter
     double KF[64];
-    DBuf<double> KFd, fdinv, fU, frhs, fr, fz, fp, fq, ftmp_e, ftmp_e2;
+    DBuf<double> KFd, fdinv, fU, frhs, fr, frp, fz, fp, fp2, fq, ftmp_e, ftmp_e2;
+    int filter_iters = 0;  // iterations of the last filter solve (when the host first reads the residual)
     Grid fg{};
 
     // SOMMG reduced-order model (tt_params.reduction): basis Q (nbasis vectors of nd), its

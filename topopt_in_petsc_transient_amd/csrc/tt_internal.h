@@ -285,6 +285,10 @@ void launch_fcg_init(Reducer& red, int bb_region, int rr_region, double rtol2, b
 void launch_fcg_update(double* x, const double* p, double* r, double* rprev, const double* q, long n, Reducer& red,
                        bool first, hipStream_t s);
 void launch_jacobi(const double* dinv, const double* r, double* z, long n, hipStream_t s);
+// Flexible-CG direction for a matrix without a fused apply kernel: beta from the z.r / z.r_prev partials
+// of launch_fcg_dots (0 on a (re)start), then p_new = z + beta p_old.
+void launch_fcg_direction(const double* z, const double* p_old, double* p_new, long n, Reducer& red, bool first,
+                          hipStream_t s);
 // power iteration: x = dinv.Ax with partials of ||x||^2 in PB_MISC; then x /= ||x||
 void launch_norm_power(const void* Ax, const void* dinv, void* x, long n, Reducer& red, bool fp64, hipStream_t s);
 void launch_scale_by_norm(void* x, long n, Reducer& red, bool fp64, hipStream_t s);

@@ -1628,6 +1628,22 @@ void launch_fcg_update(double* x, const double* p, double* r, double* rprev, con
     TT_HIP(hipGetLastError());
 }
 
+__global__ void k_fcg_direction(const double* __restrict__ z, const double* __restrict__ pold,
+                                double* __restrict__ pnew, long n, const double* __restrict__ scal, int first) {
+    const double beta = first ? 0.0 : scal[SC_BETA];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        pnew[i] = first ? z[i] : fma(beta, pold[i], z[i]);
+}
+void launch_fcg_direction(const double* z, const double* p_old, double* p_new, long n, Reducer& red, bool first,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BSR), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
+                       first ? 1 : 0);
+    TT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fcg_direction, dim3(nblocks(n)), dim3(BS), 0, s, z, p_old, p_new, n, red.scal.p,
+                       first ? 1 : 0);
+    TT_HIP(hipGetLastError());
+}
+
 __global__ void k_jacobi(const double* __restrict__ dinv, const double* __restrict__ r, double* __restrict__ z,
                          long n) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
