@@ -44,9 +44,17 @@ struct Multigrid {
     // coarsest level
     int nc = 0;
     bool dense = true;
-    DBuf<double> Ainv, AinvP, sprow, spcol, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest level
+    DBuf<double> Ainv, AinvP, Winv, sprow, spcol, cb, cx, cdinv, cx2, cd, cd2;  // FP64 coarsest level
     DBuf<int> dinfo;
-    void* blas = nullptr;  // rocblas_handle used by the rocSOLVER factorisation of the coarsest operator
+    void* blas = nullptr;  // rocblas_handle of the coarsest operator's inversion (GEMMs)
+    // the inversion's own stream, its events (assembly done, inverse done) and rocSOLVER's status words
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_asm = nullptr, ev_inv = nullptr;
+    int* info_host = nullptr;
+    bool inv_pending = false;
+    // waits (device and host) for the inverse queued by the last set-up and checks its status
+    void finish_inverse();
+    void spd_inverse(double* A, int n, int lda, double* W);
     int coarse_deg = 12;
     double clmin = 0.0, clmax = 1.0;
 

@@ -195,8 +195,11 @@ void launch_sym_solve(const double* P, const double* b, double* x, int n, double
 // (row-major n x n, n = g.nd; cleared first).
 void launch_dense_coarse(const Grid& g, const double* kf, const double* mf, double cK, double cM, const double* KE,
                          const double* M8, const uint8_t* mask, double* A, hipStream_t s);
-// Column-major lower triangle of an n x n matrix copied into its upper triangle.
-void launch_mirror_lower(double* A, int n, hipStream_t s);
+// In-place inverse of an m x m (m <= 64) symmetric positive definite block, column-major with leading
+// dimension lda (one block, Gauss-Jordan in registers); *info = 1 if a pivot is not positive.
+void launch_spd_inv_small(double* A, int m, int lda, int* info, hipStream_t s);
+// dst (cols x rows) = src (rows x cols) transposed, column-major with leading dimensions lds, ldd.
+void launch_transpose(const double* src, int rows, int cols, int lds, double* dst, int ldd, hipStream_t s);
 constexpr int AF32_WORDS = 18 * 64 + 3 * 2 * 64 * 4;  // FP32 fragments followed by the FP16-split ones
 void mfma_fragments(const double* KE, const double* M8, double* af64, float* af32, uint32_t* afh, float* hs);
 

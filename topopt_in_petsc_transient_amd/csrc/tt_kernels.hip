@@ -1271,24 +1271,81 @@ void launch_interp(const double* x, long n, int sInt, double Emin, double Emax, 
     TT_HIP(hipGetLastError());
 }
 
-__global__ void k_mirror_lower(double* A, int n) {
+// In-place inverse of an m x m (m <= 64) symmetric positive definite block (column-major, leading
+// dimension lda) by the symmetric sweep: step k, with p = 1 / a_kk, sets a_ij -= a_ik a_jk p (i, j != k),
+// a_ik *= p (i != k) and a_kk = -p; the matrix stays symmetric, and after all m steps it holds -A^-1.
+// 256 threads: thread t holds rows 16 (t / 64) .. + 15 of column t % 64 in registers; the owner lanes
+// put column k in LDS for each step (double-buffered, one barrier per step).  A pivot that is not
+// positive sets *info.
+__global__ __launch_bounds__(256) void k_spd_inv_small(double* A, int m, int lda, int* info) {
+    __shared__ double colk[2][64];
+    const int t = threadIdx.x, j = t & 63;
+    const int rb = __builtin_amdgcn_readfirstlane(t >> 6);  // (wave-uniform: the pivot row's wave branches)
+    double c[16];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+        const int i = 16 * rb + ii;
+        c[ii] = (i < m && j < m) ? A[i + (size_t)j * lda] : 0.0;
+    }
+    bool bad = false;
+    for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const int k = 16 * kb + kk;
+            if (k < m) {
+                double* ck = colk[k & 1];
+                if (j == k)
+#pragma unroll
+                    for (int ii = 0; ii < 16; ii++) ck[16 * rb + ii] = c[ii];
+                __syncthreads();
+                const double akk = ck[k];
+                bad = bad || !(akk > 0.0);
+                const double p = 1.0 / akk;
+                const double ajk = ck[j] * p;  // a_jk / a_kk (= a_kj / a_kk)
+                if (j != k) {
+#pragma unroll
+                    for (int ii = 0; ii < 16; ii++)
+                        c[ii] = (rb == kb && ii == kk) ? ajk : fma(-ck[16 * rb + ii], ajk, c[ii]);
+                } else {
+#pragma unroll
+                    for (int ii = 0; ii < 16; ii++)
+                        c[ii] = (rb == kb && ii == kk) ? -p : ck[16 * rb + ii] * p;
+                }
+            }
+        }
+    }
+    if (bad && t == 0) *info = 1;
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) {
+        const int i = 16 * rb + ii;
+        if (i < m && j < m) A[i + (size_t)j * lda] = -c[ii];
+    }
+}
+void launch_spd_inv_small(double* A, int m, int lda, int* info, hipStream_t s) {
+    TT_CHECK(m >= 1 && m <= 64, -3, "launch_spd_inv_small: block of 1..64");  // TT_ERR_INTERNAL
+    hipLaunchKernelGGL(k_spd_inv_small, dim3(1), dim3(256), 0, s, A, m, lda, info);
+    TT_HIP(hipGetLastError());
+}
+
+// dst (cols x rows) = src (rows x cols)^T, both column-major with their leading dimensions
+__global__ void k_transpose(const double* __restrict__ src, int rows, int cols, int lds, double* __restrict__ dst,
+                            int ldd) {
     __shared__ double t[32][33];
-    const int bi = blockIdx.x, bj = blockIdx.y;  // tile (bi, bj) of the lower part, bi >= bj
-    if (bi < bj) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;   // src tile: rows i0.., cols j0..
     for (int r = ty; r < 32; r += 8) {
-        const int i = bi * 32 + tx, j = bj * 32 + r;  // element (row i, col j) at A[i + j n]
-        t[r][tx] = (i < n && j < n) ? A[(size_t)j * n + i] : 0.0;
+        const int i = i0 + tx, j = j0 + r;
+        if (i < rows && j < cols) t[r][tx] = src[i + (size_t)j * lds];
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
-        const int j = bj * 32 + tx, i = bi * 32 + r;  // write (row j, col i) = (row i, col j)
-        if (i < n && j < n && i > j) A[(size_t)i * n + j] = t[tx][r];
+        const int i = i0 + r, j = j0 + tx;  // dst (row j, col i) = src (row i, col j)
+        if (i < rows && j < cols) dst[j + (size_t)i * ldd] = t[tx][r];
     }
 }
-void launch_mirror_lower(double* A, int n, hipStream_t s) {
-    const int nt = (n + 31) / 32;
-    hipLaunchKernelGGL(k_mirror_lower, dim3(nt, nt), dim3(256), 0, s, A, n);
+void launch_transpose(const double* src, int rows, int cols, int lds, double* dst, int ldd, hipStream_t s) {
+    hipLaunchKernelGGL(k_transpose, dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, s, src, rows, cols, lds,
+                       dst, ldd);
     TT_HIP(hipGetLastError());
 }
 // ---- dense coarsest operator, assembled on the device: entry (i, j) of N (cK K + cM M) N + I - N, i = 3 p + c,

@@ -19,7 +19,6 @@
 #include <cstring>
 
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "tt_context.h"
 
@@ -39,6 +38,15 @@ static void build_mask(const Grid& g, int boundaries, std::vector<uint8_t>& m) {
                 }
                 m[i + (long)g.nx * (j + (long)g.ny * k)] = b;
             }
+}
+
+// Split of an order-n block for the Schur-complement inverse: the leading block a multiple of 64 near n / 2.
+static int spd_split(int n) { return std::min(((n / 2 + 63) / 64) * 64, n - 1); }
+// workspace (doubles) of spd_inverse at order n: T of the top split, and the trailing block's own
+static size_t spd_workspace(int n) {
+    if (n <= 64) return 0;
+    const int n1 = spd_split(n), n2 = n - n1;
+    return std::max(spd_workspace(n1), (size_t)n1 * n2 + spd_workspace(n2));
 }
 
 void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
@@ -139,10 +147,16 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
             spcol.alloc(ntiles * 64);
         }
         dinfo.alloc(2);
+        Winv.alloc(spd_workspace(nc));
+        // the inversion runs on a stream of its own, beside the smoother set-up and the initial state
+        TT_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        TT_HIP(hipEventCreateWithFlags(&ev_asm, hipEventDisableTiming));
+        TT_HIP(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming));
+        TT_HIP(hipHostMalloc((void**)&info_host, 2 * sizeof(int), hipHostMallocDefault));
         rocblas_handle h = nullptr;
         TT_CHECK(rocblas_create_handle(&h) == rocblas_status_success, TT_ERR_HIP, "rocblas_create_handle failed");
         blas = h;
-        TT_CHECK(rocblas_set_stream(h, ctx->stream) == rocblas_status_success, TT_ERR_HIP, "rocblas_set_stream failed");
+        TT_CHECK(rocblas_set_stream(h, s2) == rocblas_status_success, TT_ERR_HIP, "rocblas_set_stream failed");
     } else {
         cdinv.alloc(nc);
         cx2.alloc(nc);
@@ -201,7 +215,12 @@ double Multigrid::power_lmax(int l, int iters) {
 }
 
 Multigrid::~Multigrid() {
+    if (s2) (void)hipStreamSynchronize(s2);
     if (blas) (void)rocblas_destroy_handle((rocblas_handle)blas);
+    if (info_host) (void)hipHostFree(info_host);
+    if (ev_asm) (void)hipEventDestroy(ev_asm);
+    if (ev_inv) (void)hipEventDestroy(ev_inv);
+    if (s2) (void)hipStreamDestroy(s2);
 }
 
 void Multigrid::assemble_coarse_inverse() {
@@ -214,24 +233,63 @@ void Multigrid::assemble_coarse_inverse() {
         md = ctx->mf.p;
     }
     const int n = nc;
-    // the dense coarsest operator, assembled on the device from the level's FP64 coefficients
-    launch_dense_coarse(g, kd, md, cK, cM, lc.KE64.p, lc.M864.p, lc.mask.p, Ainv.p, ctx->stream);
-    // A^-1 on the device: Cholesky (potrf) then the inverse from the factor (potri), lower
-    // triangle in column-major order, mirrored into the upper triangle for the GEMV.
     hipStream_t s = ctx->stream;
+    finish_inverse();  // (an inversion still in flight reads the matrix assembled below)
+    // the dense coarsest operator, assembled on the device from the level's FP64 coefficients
+    launch_dense_coarse(g, kd, md, cK, cM, lc.KE64.p, lc.M864.p, lc.mask.p, Ainv.p, s);
+    // A^-1 on the second stream, while the main stream sets up the smoothers and computes the initial
+    // state (spd_inverse), then packed; the status word goes to pinned host memory and is checked where
+    // the inverse is first needed (finish_inverse)
+    TT_HIP(hipEventRecord(ev_asm, s));
+    TT_HIP(hipStreamWaitEvent(s2, ev_asm, 0));
+    TT_HIP(hipMemsetAsync(dinfo.p, 0, 2 * sizeof(int), s2));
+    spd_inverse(Ainv.p, n, n, Winv.p);
+    launch_sym_pack(Ainv.p, n, AinvP.p, s2);
+    TT_HIP(hipMemcpyAsync(info_host, dinfo.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s2));
+    TT_HIP(hipEventRecord(ev_inv, s2));
+    inv_pending = true;
+}
+
+// In-place inverse of the SPD matrix A (order n, column-major, leading dimension lda) on the inversion
+// stream, by Schur complements:
+//     [A11 A12; A21 A22]^-1 = [X11 + T X22 T', -T X22; -X22 T', X22],  X11 = A11^-1, T = X11 A12,
+//     X22 = S^-1 with S = A22 - A21 T (SPD when A is),
+// recursively down to blocks of at most 64, which one wave inverts in LDS (launch_spd_inv_small); the rest
+// is 4 rocBLAS GEMMs per split (4/3 n^3 flops in all) and a transpose for the lower block.
+void Multigrid::spd_inverse(double* A, int n, int lda, double* W) {
+    if (n <= 64) {
+        launch_spd_inv_small(A, n, lda, dinfo.p, s2);
+        return;
+    }
+    const int n1 = spd_split(n), n2 = n - n1;
+    double* A11 = A;
+    double* A21 = A + n1;
+    double* A12 = A + (size_t)n1 * lda;
+    double* A22 = A12 + n1;
+    spd_inverse(A11, n1, lda, W);  // (its workspace is free again before T is written: same stream)
     rocblas_handle h = (rocblas_handle)blas;
-    // (both status words are read back once, at the end: one synchronisation for the whole set-up)
-    TT_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p) == rocblas_status_success, TT_ERR_HIP,
-             "rocsolver_dpotrf failed");
-    TT_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, n, Ainv.p, n, dinfo.p + 1) == rocblas_status_success, TT_ERR_HIP,
-             "rocsolver_dpotri failed");
-    launch_mirror_lower(Ainv.p, n, s);
-    launch_sym_pack(Ainv.p, n, AinvP.p, s);
-    int info[2] = {0, 0};
-    TT_HIP(hipMemcpyAsync(info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, s));
-    TT_HIP(hipStreamSynchronize(s));
-    TT_CHECK(info[0] == 0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
-    TT_CHECK(info[1] == 0, TT_ERR_INTERNAL, "coarsest operator inverse failed");
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    double* T = W;  // n1 x n2, leading dimension n1
+    auto gemm = [&](rocblas_operation tb, int m, int nn, int k, const double* alpha, const double* a, int la,
+                    const double* b, int lb, const double* beta, double* c, int lc) {
+        TT_CHECK(rocblas_dgemm(h, rocblas_operation_none, tb, m, nn, k, alpha, a, la, b, lb, beta, c, lc) ==
+                     rocblas_status_success,
+                 TT_ERR_HIP, "rocblas_dgemm failed");
+    };
+    gemm(rocblas_operation_none, n1, n2, n1, &one, A11, lda, A12, lda, &zero, T, n1);     // T = X11 A12
+    gemm(rocblas_operation_none, n2, n2, n1, &mone, A21, lda, T, n1, &one, A22, lda);     // S = A22 - A21 T
+    spd_inverse(A22, n2, lda, W + (size_t)n1 * n2);                                          // X22 = S^-1
+    gemm(rocblas_operation_none, n1, n2, n2, &mone, T, n1, A22, lda, &zero, A12, lda);    // X12 = -T X22
+    gemm(rocblas_operation_transpose, n1, n1, n2, &mone, A12, lda, T, n1, &one, A11, lda);  // X11 -= X12 T'
+    launch_transpose(A12, n1, n2, lda, A21, lda, s2);                                       // X21 = X12'
+}
+
+void Multigrid::finish_inverse() {
+    if (!inv_pending) return;
+    inv_pending = false;
+    TT_HIP(hipStreamWaitEvent(ctx->stream, ev_inv, 0));
+    TT_HIP(hipEventSynchronize(ev_inv));
+    TT_CHECK(info_host[0] == 0, TT_ERR_INTERNAL, "coarsest operator is not positive definite");
 }
 
 std::vector<char**> Multigrid::rotating() {
@@ -399,6 +457,7 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
 }
 
 void Multigrid::apply_fcg(const double* r, const double* rp, double* z, bool first) {
+    finish_inverse();
     if (L.size() == 1) {
         apply(r, z);
         launch_fcg_dots(z, r, rp, L[0]->g.nd, ctx->red, first, ctx->stream);
@@ -408,6 +467,7 @@ void Multigrid::apply_fcg(const double* r, const double* rp, double* z, bool fir
 }
 
 void Multigrid::apply(const double* r, double* z) {
+    finish_inverse();
     MGLevel& l0 = *L[0];
     hipStream_t s = ctx->stream;
     const long nd = l0.g.nd;

@@ -62,6 +62,7 @@ void Context::complete_initial_state() {
     const double wM[3] = {0.0, -prm.dalpha, 0.0};
     launch_newmark_rhs(op_fine64, s0, s0 + nd, s0 + 2 * nd, F.p, load_intensity(0), wK, wM, rhs.p, stream);
     solve_M(rhs.p, s0 + 2 * nd);
+    mg.finish_inverse();  // the coarsest inverse, computed beside the smoother set-up and this solve
     for (auto& v : slot_step) v = -1;
     slot_step[0] = 0;
     initial_done = true;
