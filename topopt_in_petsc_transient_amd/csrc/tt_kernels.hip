@@ -1302,15 +1302,13 @@ __global__ __launch_bounds__(256) void k_spd_inv_small(double* A, int m, int lda
                 bad = bad || !(akk > 0.0);
                 const double p = 1.0 / akk;
                 const double ajk = ck[j] * p;  // a_jk / a_kk (= a_kj / a_kk)
-                if (j != k) {
+                // c_i -= a_ik a_jk / a_kk; column k itself becomes a_ik p; then the pivot row's element
 #pragma unroll
-                    for (int ii = 0; ii < 16; ii++)
-                        c[ii] = (rb == kb && ii == kk) ? ajk : fma(-ck[16 * rb + ii], ajk, c[ii]);
-                } else {
+                for (int ii = 0; ii < 16; ii++) c[ii] = fma(-ck[16 * rb + ii], ajk, c[ii]);
+                if (j == k)
 #pragma unroll
-                    for (int ii = 0; ii < 16; ii++)
-                        c[ii] = (rb == kb && ii == kk) ? -p : ck[16 * rb + ii] * p;
-                }
+                    for (int ii = 0; ii < 16; ii++) c[ii] = ck[16 * rb + ii] * p;
+                if (rb == kb) c[kk] = j == k ? -p : ajk;
             }
         }
     }

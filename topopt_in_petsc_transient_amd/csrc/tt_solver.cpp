@@ -305,6 +305,8 @@ void Multigrid::setup_operator(double cK_, double cM_) {
     cM = cM_;
     hipStream_t s = ctx->stream;
     const int nl = (int)L.size();
+    // the coarsest inverse first: it runs on its own stream beside the power iterations below
+    if (dense) assemble_coarse_inverse();
     for (int l = 0; l < nl - 1; l++) {
         MGLevel& lv = *L[l];
         launch_diag_inv(lv.op, cK, cM, lv.dinv.p, lv.fp64, s);
@@ -312,9 +314,7 @@ void Multigrid::setup_operator(double cK_, double cM_) {
         if (l < 8) ctx->stats.lambda_max[l] = lv.lmax;
     }
     ctx->stats.nlvls = nl;
-    if (dense) {
-        assemble_coarse_inverse();
-    } else {
+    if (!dense) {
         // Chebyshev coarse solve in FP64 on [clmax/30, 1.05 clmax] of D^-1 A
         OpLevel op = coarse_op64();
         launch_diag_inv(op, cK, cM, cdinv.p, true, s);
