@@ -42,7 +42,7 @@ MFMA_EXEC_A32 = 520058 * 16384
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, FETCH scaled by 1.99922, the factor the
 # counters read for a device copy of 64 MiB; WRITE by 1.0)
 TRAFFIC_A32 = 20985242     # FP32 A apply: FETCH 7050.6 KiB, WRITE 6397.7 KiB
-TRAFFIC_FCG64 = 74640599   # FP64 flexible-CG apply: FETCH 23576.2 KiB, WRITE 25757.2 KiB
+TRAFFIC_FCG64 = 84601994   # FP64 flexible-CG apply: FETCH 28416.6 KiB, WRITE 25808.1 KiB
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
@@ -344,19 +344,21 @@ def main():
         cpu = cpu_baseline(x, args.cpu_steps)
 
     # roofline of the dominant kernel of the step: the FP64 flexible-CG apply of every Krylov iteration
-    # (p = z + beta p_old formed while staging, q = A p, p.q partials), the largest single kernel of the
-    # mixed step.  Algorithmic work per launch: the stacked [K; M] 48x24 element contraction,
-    # 2 x 1152 flops per element; its algorithmic bytes: z, p_old read, p, q written (FP64), kfac, mfac, mask.
+    # (z staged, A z contracted, p = z + beta p_old and q = A z + beta q_old = A p formed at the owned
+    # nodes, p.q partials), the largest single kernel of the mixed step.  Algorithmic work per launch: the
+    # stacked [K; M] 48x24 element contraction, 2 x 1152 flops per element; its algorithmic bytes: z, p_old,
+    # q_old read, p, q written (FP64), kfac, mfac, mask.
     fl = 2 * 1152 * ops["A"]["nel"]
     nd0, nel0 = ops["A"]["nd"], ops["A"]["nel"]
     fcg_ms = ops["fcg_fp64_ms"]
     ach = fl / (fcg_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": ach / (PEAK_FP64 / 1e12), "traffic": TRAFFIC_FCG64,
-            "kernel": "k_apply_tiles<double,double,4,true,StageFcg<double>,EpiFcg> (flexible-CG apply: p = z + beta "
-                      "p_old, q = A p, p.q partials; FP64, 128x64x64, 8x8x4-node tiles, v_mfma_f64_16x16x4f64)",
+            "kernel": "k_apply_tiles<double,double,4,true,StageOne<double>,EpiFcg> (flexible-CG apply: z staged, p = "
+                      "z + beta p_old, q = A z + beta q_old = A p, p.q partials; FP64, 128x64x64, 8x8x4-node tiles, "
+                      "v_mfma_f64_16x16x4f64)",
             "kernel_ms": fcg_ms, "flops_per_launch": fl,
-            "algorithmic_bytes_per_launch": 8 * (4 * nd0 + 2 * nel0) + nd0 // 3}
+            "algorithmic_bytes_per_launch": 8 * (5 * nd0 + 2 * nel0) + nd0 // 3}
     # second entry: the FP32 Newmark-operator apply of the V-cycle smoothers (FP16-split matrix cores)
     a32 = ops["A"]
     ach32 = fl / (a32["fp32_ms"] * 1e-3) / 1e12

@@ -165,7 +165,7 @@ Context::Context(const tt_params& p) : prm(p) {
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
 
-    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &dinvM, &kx, &lam, &lamh[0], &lamh[1], &lamh[2], &lamh[3], &lamh[4], &lamh[5], &lamT, &mu, &nu, &mun, &nun, &hdu,
+    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &kq2, &dinvM, &kx, &lam, &lamh[0], &lamh[1], &lamh[2], &lamh[3], &lamh[4], &lamh[5], &lamT, &mu, &nu, &mun, &nun, &hdu,
                             &hddu, &hdddu, &Kl, &Ml, &rhs, &nu0})
         b->alloc(nd);
     TT_HIP(hipMemsetAsync(lamT.p, 0, nd * 8, stream));
@@ -588,19 +588,21 @@ int tt_time_fcg_apply(tt_ctx* ctx, int nrep, double* ms) {
     TT_CHECK(C.density_set, TT_ERR_STATE, "tt_set_density must be called first");
     TT_CHECK(ms && nrep > 0, TT_ERR_ARG, "bad tt_time_fcg_apply arguments");
     const long nd = C.g.nd;
-    // the flexible-CG apply of a Krylov iteration on the context's own vectors: z and p_old from a
+    // the flexible-CG apply of a Krylov iteration on the context's own vectors: z, p_old and q_old from a
     // deterministic hash, beta = 0.5, then nrep launches of the tile kernel (the beta kernel excluded)
     tt::launch_fill_hash(C.kz.p, nd, 0x51ED270B27A1F00Dull, true, C.stream);
     tt::launch_fill_hash(C.kp0.p, nd, 0x2F1C3A5B4D6E7F81ull, true, C.stream);
     tt::launch_fill(C.red.scal.p + tt::SC_BETA, 0.5, 1, true, C.stream);
-    tt::launch_fcg_apply(C.op_fine64, C.cK_A, C.cM_A, C.kz.p, C.kp0.p, C.kp1.p, C.kq.p, C.red, false, C.stream, false);
+    tt::launch_fill_hash(C.kq2.p, nd, 0x7C3B1A9E5D2F4681ull, true, C.stream);
+    tt::launch_fcg_apply(C.op_fine64, C.cK_A, C.cM_A, C.kz.p, C.kp0.p, C.kp1.p, C.kq2.p, C.kq.p, C.red, false, C.stream,
+                         false);
     hipEvent_t e0, e1;
     TT_HIP(hipEventCreate(&e0));
     TT_HIP(hipEventCreate(&e1));
     TT_HIP(hipEventRecord(e0, C.stream));
     for (int i = 0; i < nrep; i++)
-        tt::launch_fcg_apply(C.op_fine64, C.cK_A, C.cM_A, C.kz.p, C.kp0.p, C.kp1.p, C.kq.p, C.red, false, C.stream,
-                             false);
+        tt::launch_fcg_apply(C.op_fine64, C.cK_A, C.cM_A, C.kz.p, C.kp0.p, C.kp1.p, C.kq2.p, C.kq.p, C.red, false,
+                             C.stream, false);
     TT_HIP(hipEventRecord(e1, C.stream));
     TT_HIP(hipEventSynchronize(e1));
     float t = 0.0f;

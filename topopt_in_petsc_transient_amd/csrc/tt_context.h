@@ -124,7 +124,7 @@ struct Context {
     Multigrid mg;
 
     // Krylov work vectors (FP64, nd); kx holds the solution of a graph-replayed solve
-    DBuf<double> kr, krp, kz, kp0, kp1, kq, dinvM, kx;
+    DBuf<double> kr, krp, kz, kp0, kp1, kq, kq2, dinvM, kx;
 
     // One multigrid-preconditioned flexible-CG iteration captured as a HIP graph, keyed by everything
     // its launches bake in (pointers of the rotating buffers, first-iteration flag, operator scalars),

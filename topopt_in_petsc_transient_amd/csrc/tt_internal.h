@@ -228,10 +228,11 @@ void launch_newmark_rhs(const OpLevel& L, const double* u, const double* v, cons
 // diagonal of (cK K + cM M) with identity on clamped dofs, inverted
 void launch_diag_inv(const OpLevel& L, double cK, double cM, void* dinv, bool fp64, hipStream_t s);
 
-// FCG fused apply: p = z + beta*p_old (beta from scal), q = A p, partial p.q -> alpha = gamma / pq.
+// FCG fused apply: p = z + beta p_old (beta from scal) and q = A p = A z + beta q_old, partial p.q ->
+// alpha = gamma / pq.  Only z is staged with its halo; p_old and q_old are read at the owned nodes.
 // beta = false: the tile kernel alone, with the beta already in scal (measurement)
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
-                      double* q, Reducer& red, bool first, hipStream_t s, bool beta = true);
+                      const double* q_old, double* q_new, Reducer& red, bool first, hipStream_t s, bool beta = true);
 
 // vector kernels
 void launch_fill(void* x, double v, long n, bool fp64, hipStream_t s);

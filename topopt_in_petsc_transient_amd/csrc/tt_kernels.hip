@@ -224,18 +224,6 @@ struct StageOne {  // w = u
     __device__ __forceinline__ T k(unsigned i) const { return ld(u, i); }
     __device__ __forceinline__ T m(unsigned i) const { return ld(u, i); }
 };
-// Flexible-CG direction p = z + beta p_old, beta = z.(r - r_prev) / gamma_old computed by k_fcg_beta.
-template <typename T>
-struct StageFcg {
-    const T* z;
-    const T* pold;
-    const double* scal;
-    int first;
-    T beta;
-    __device__ __forceinline__ void prologue() { beta = first ? (T)0 : (T)scal[SC_BETA]; }
-    __device__ __forceinline__ T k(unsigned i) const { return first ? ld(z, i) : ld(z, i) + beta * ld(pold, i); }
-    __device__ __forceinline__ T m(unsigned i) const { return k(i); }
-};
 // First Chebyshev iterate x0 = c1 D^-1 b formed on the fly, with b = (T) r (FP64) on the finest level
 template <typename T, bool FROM64>
 struct StageChebFirst {
@@ -666,22 +654,42 @@ struct EpiRHS {  // R = N.(fs F + yK + yM)
     TT_EPI_CALL
 };
 
-struct EpiFcg {  // p_new = staged p (owned node), q = A p, partials of p.q
+// Flexible-CG direction and its product, with z staged (A z from the contraction): p = z + beta p_old,
+// q = A p = A z + beta q_old (p_old, q_old read at the owned node), partials of p.q.  beta from k_fcg_beta
+// (0 on a (re)start).
+struct EpiFcg {
     double* pnew;
     double* q;
+    const double* pold;
+    const double* qold;
+    const double* scal;
+    int first;
     double* part;
-    TT_NO_PRE
+    struct Pre {
+        double p[3], q[3], beta;
+    };
+    template <class S>
+    __device__ __forceinline__ void fetch(int n, const S&, Pre& pr) const {
+        pr.beta = first ? 0.0 : scal[SC_BETA];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            pr.p[c] = first ? 0.0 : pold[3 * n + c];
+            pr.q[c] = first ? 0.0 : qold[3 * n + c];
+        }
+    }
     template <class S>
     __device__ void put(int n, uint8_t cl, const double (&yK)[3], const double (&yM)[3], const S& st,
-                        const double (&uo)[3], double (&acc)[2], const Pre&) const {
+                        const double (&uo)[3], double (&acc)[2], const Pre& pr) const {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int i = 3 * n + c;
-            const double p = input_at(st, cl, c, i, uo);
-            const double v = ((cl >> c) & 1) ? p : yK[c] + yM[c];
+            const double z = input_at(st, cl, c, i, uo);
+            const double az = ((cl >> c) & 1) ? z : yK[c] + yM[c];
+            const double p = first ? z : fma(pr.beta, pr.p[c], z);
+            const double v = first ? az : fma(pr.beta, pr.q[c], az);
             pnew[i] = p;
             q[i] = v;
-            acc[0] += p * v;
+            acc[0] = fma(p, v, acc[0]);
         }
     }
     __device__ void done(double (&acc)[2]) const {
@@ -936,16 +944,15 @@ constexpr int BSR = 1024;
 __global__ __launch_bounds__(BSR) void k_fcg_beta(const double* __restrict__ dots_part, int dots_nblk, double* scal,
                                                   int first);
 void launch_fcg_apply(const OpLevel& L, double cK, double cM, const double* z, const double* p_old, double* p_new,
-                      double* q, Reducer& red, bool first, hipStream_t s, bool beta) {
+                      const double* q_old, double* q_new, Reducer& red, bool first, hipStream_t s, bool beta) {
     TT_CHECK(apply_nblocks(L.g) <= red.maxblocks, -3, "reducer too small");
     if (beta) {
         hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(BSR), 0, s, red.part(PB_DOTS), red.nblk[PB_DOTS], red.scal.p,
                            first ? 1 : 0);
         TT_HIP(hipGetLastError());
     }
-    StageFcg<double> st{z, p_old, red.scal.p, first ? 1 : 0, 0.0};
-    EpiFcg e{p_new, q, red.part(PB_PQ)};
-    red.nblk[PB_PQ] = run_apply<double, false, true>(L, cK, cM, st, e, s);
+    EpiFcg e{p_new, q_new, p_old, q_old, red.scal.p, first ? 1 : 0, red.part(PB_PQ)};
+    red.nblk[PB_PQ] = run_apply<double, false, true>(L, cK, cM, StageOne<double>{z}, e, s);
     red.nv[PB_PQ] = 1;
 }
 

@@ -539,6 +539,8 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
     bool first = true, conv = false;
     double* pold = kp0.p;
     double* pnew = kp1.p;
+    double* qold = kq2.p;
+    double* qnew = kq.p;
     // A multigrid-preconditioned iteration is replayed as a HIP graph; its solution vector is then
     // the fixed kx (copied in and out) so that one graph serves every solve.
     const bool graphed = use_mg && prm.graphs;
@@ -551,14 +553,15 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
             launch_jacobi(dinvM.p, kr.p, kz.p, nd, stream);
             launch_fcg_dots(kz.p, kr.p, krp.p, nd, red, first, stream);
         }
-        launch_fcg_apply(op, cK, cM, kz.p, pold, pnew, kq.p, red, first, stream);
-        launch_fcg_update(xs, pnew, kr.p, krp.p, kq.p, nd, red, first, stream);
+        launch_fcg_apply(op, cK, cM, kz.p, pold, pnew, qold, qnew, red, first, stream);
+        launch_fcg_update(xs, pnew, kr.p, krp.p, qnew, nd, red, first, stream);
     };
     int it = 0;
     while (!conv && it < prm.maxit) {
         if (graphed) {
             const std::vector<char**> rot = mg.rotating();
-            std::vector<uintptr_t> key = {(uintptr_t)first, (uintptr_t)pold, (uintptr_t)pnew};
+            std::vector<uintptr_t> key = {(uintptr_t)first, (uintptr_t)pold, (uintptr_t)pnew, (uintptr_t)qold,
+                                          (uintptr_t)qnew};
             uint64_t cb[2];
             std::memcpy(cb, &cK, 8);
             std::memcpy(cb + 1, &cM, 8);
@@ -596,6 +599,7 @@ SolveResult Context::fcg(double cK, double cM, bool use_mg, SolveKind kind, cons
             iteration();
         }
         std::swap(pold, pnew);
+        std::swap(qold, qnew);
         first = false;
         it++;
         if (it < first_look) continue;
