@@ -112,7 +112,7 @@ def test_reduced_gradient_is_the_fixed_basis_derivative(objective):
         rs = O.reduced_forward(p, Kr, Mr, Cr, Pr)
         return O.reduced_objective_gradient(p, xx, Q, Kr, Mr, Cr, rs)[0]
 
-    assert abs(f(x) - r["fx"]) <= 1e-13 * abs(r["fx"])  # (threaded dot products: not bitwise)
+    assert abs(f(x) - r["fx"]) <= 1e-12 * abs(r["fx"])  # (threaded BLAS reductions: not bitwise, ~1e-13 apart)
     h = 1e-6
     for e in (0, 17, 40, O.nel(p) - 1):
         xp, xm = x.copy(), x.copy()

@@ -211,7 +211,12 @@ struct Context {
     void complete_initial_state();
     void forward();
     void objective_gradient(double* fx_host, double* dfdx_dev);
-    void newmark_step(const double* s_in, double* s_out, int t);
+    // one Newmark step from state s_in to s_out; with next_out, the corrector also writes the next step's
+    // initial guess there (the next step's s_out), and that step skips its own extrapolation
+    void newmark_step(const double* s_in, double* s_out, int t, double* next_out = nullptr);
+    Extrap forward_guess(int t, const double* s_in) const;
+    int guess_t = -1;               // the step whose initial guess a corrector has written ahead (-1: none)
+    double* guess_out = nullptr;    // and where
     double load_intensity(int step) const;
 
     // filter

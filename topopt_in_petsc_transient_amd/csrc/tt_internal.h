@@ -299,9 +299,13 @@ void launch_scale_by_norm(void* x, long n, Reducer& red, bool fp64, hipStream_t 
 
 // Newmark corrector: vn = a0 v + a1 a + a2 (un - u); an = -a3 v - a4 a + a5 (un - u).  With nh > 0 it
 // also writes the nh vectors hout[0] = a, hout[j] = hin[j - 1] (each n long, stored n apart).
+struct Extrap;
+// With next (its coefficients; its vectors are ignored) and gout, the kernel also writes the next step's
+// initial guess sum_k c_k w_k over w = (un, vn, an, a, hin[0], hin[1]) (the new state's u, v, a and history),
+// in k_extrapolate's order of operations.
 void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,
                            double* an, const double* c, long n, const double* hin, double* hout, int nh,
-                           hipStream_t s);
+                           hipStream_t s, const Extrap* next = nullptr, double* gout = nullptr);
 // adjoint combination (see transient.cpp)
 void launch_adjoint_combine(const double* Kl, const double* Ml, const double* hdu, const double* hddu,
                             const double* hdddu, const double* mun, const double* nun, const uint8_t* mask,

@@ -1773,31 +1773,53 @@ void launch_scale(double* x, long n, double a, hipStream_t s) {
 // ------------------------------------------------------------------ Newmark / adjoint vector kernels
 // corrector (LinearElasticity.cc:1363-1375): c = a[0..5].  The acceleration history of the new state
 // is shifted in the same pass: hout[0] = a, hout[j] = hin[j - 1] (j < nh), vectors n apart.
-__global__ void k_newmark_update(const double* __restrict__ u, const double* __restrict__ v,
+struct GuessCoef {
+    double c[6];
+    int m;
+};
+__global__ void k_newmark_update(const double* u, const double* __restrict__ v,
                                  const double* __restrict__ a, const double* __restrict__ un, double* __restrict__ vn,
                                  double* __restrict__ an, double a0, double a1, double a2, double a3, double a4,
-                                 double a5, long n, const double* __restrict__ hin, double* __restrict__ hout, int nh) {
+                                 double a5, long n, const double* __restrict__ hin, double* __restrict__ hout, int nh,
+                                 GuessCoef gc, double* gout) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (nh > 0) hout[i] = a[i];
+    const double ai = a[i];
+    if (nh > 0) hout[i] = ai;
     for (int j = 1; j < nh; j++) hout[j * n + i] = hin[(j - 1) * n + i];
-    const double du = un[i] - u[i];
+    const double uni = un[i];
+    const double du = uni - u[i];
     double vv = 0.0;
     vv += a0 * v[i];
-    vv += a1 * a[i];
+    vv += a1 * ai;
     vv += a2 * du;
     double aa = 0.0;
     aa += -a3 * v[i];
-    aa += -a4 * a[i];
+    aa += -a4 * ai;
     aa += a5 * du;
     vn[i] = vv;
     an[i] = aa;
+    if (gout) {  // the next step's guess (gout may be this step's input u: u[i] is already read)
+        double s = gc.c[0] * uni;
+        if (gc.m > 1) s = fma(gc.c[1], vv, s);
+        if (gc.m > 2) s = fma(gc.c[2], aa, s);
+        if (gc.m > 3) s = fma(gc.c[3], ai, s);
+        if (gc.m > 4) s = fma(gc.c[4], hin[i], s);
+        if (gc.m > 5) s = fma(gc.c[5], hin[n + i], s);
+        gout[i] = s;
+    }
 }
 void launch_newmark_update(const double* u, const double* v, const double* a, const double* un, double* vn,
                            double* an, const double* c, long n, const double* hin, double* hout, int nh,
-                           hipStream_t s) {
+                           hipStream_t s, const Extrap* next, double* gout) {
+    GuessCoef gc{};
+    if (next) {
+        TT_CHECK(next->m >= 1 && next->m <= 6 && nh >= 3, -3, "launch_newmark_update: guess of 1..6 terms");
+        for (int k = 0; k < next->m; k++) gc.c[k] = next->c[k];
+        gc.m = next->m;
+    }
     hipLaunchKernelGGL(k_newmark_update, dim3(nblocks(n)), dim3(BS), 0, s, u, v, a, un, vn, an, c[0], c[1], c[2],
-                       c[3], c[4], c[5], n, hin, hout, nh);
+                       c[3], c[4], c[5], n, hin, hout, nh, gc, next ? gout : (double*)nullptr);
     TT_HIP(hipGetLastError());
 }
 

@@ -69,7 +69,32 @@ void Context::complete_initial_state() {
     forward_done = false;
 }
 
-void Context::newmark_step(const double* s_in, double* s_out, int t) {
+// Initial guess of step t from state t-1 (s_in): the displacement for which the corrector returns the
+// acceleration predicted by the polynomial through the last m = min(t, kFwdHist) accelerations a_{t-1} ..
+// a_{t-m} (held in the state), u_{t-1} + (a3 v_{t-1} + a4 a_{t-1} + a^_t) / a5; with m = 1 it is the
+// constant-acceleration step u + dt v + dt^2/2 a (Newmark).  The reference starts from u_{t-1} (:1358); the
+// converged answer is the same to the solver tolerance.  Measured on the 128x64x64 sweep: 9.1 Krylov
+// iterations per step from u_{t-1}, 5.2 with m = 1, 2.1 with m = 4.
+Extrap Context::forward_guess(int t, const double* s_in) const {
+    const long nd = g.nd;
+    const int m = std::min(t, kFwdHist);
+    Extrap e{};
+    e.m = 2 + m;
+    e.v[0] = s_in;
+    e.c[0] = 1.0;
+    e.v[1] = s_in + nd;
+    e.c[1] = a[3] / a[5];
+    long bin = 1;
+    for (int k = 1; k <= m; k++) {
+        bin = bin * (m - k + 1) / k;
+        e.v[1 + k] = s_in + (size_t)(1 + k) * nd;  // a_{t-k}
+        e.c[1 + k] = (k % 2 ? 1.0 : -1.0) * (double)bin / a[5];
+    }
+    e.c[2] += a[4] / a[5];
+    return e;
+}
+
+void Context::newmark_step(const double* s_in, double* s_out, int t, double* next_out) {
     const long nd = g.nd;
     const double* u = s_in;
     const double* v = s_in + nd;
@@ -81,32 +106,22 @@ void Context::newmark_step(const double* s_in, double* s_out, int t) {
     const double wK[3] = {a[2] * be, -a[0] * be, -a[1] * be};
     const double wM[3] = {a[5] + a[2] * al, a[3] - a[0] * al, a[4] - a[1] * al};
     launch_newmark_rhs(op_fine64, u, v, ac, F.p, load_intensity(t), wK, wM, rhs.p, stream);
-    // Initial guess: the displacement for which the corrector returns the acceleration predicted by
-    // the polynomial through the last m = min(t, kFwdHist) accelerations a_{t-1} .. a_{t-m} (held in
-    // the state), u_{t-1} + (a3 v_{t-1} + a4 a_{t-1} + a^_t) / a5; with m = 1 it is the constant-
-    // acceleration step u + dt v + dt^2/2 a (Newmark).  The reference starts from u_{t-1} (:1358); the
-    // converged answer is the same to the solver tolerance.  Measured on the 128x64x64 sweep: 9.1
-    // Krylov iterations per step from u_{t-1}, 5.2 with m = 1, 2.1 with m = 4.
-    {
-        const int m = std::min(t, kFwdHist);
-        Extrap e{};
-        e.m = 2 + m;
-        e.v[0] = u;
-        e.c[0] = 1.0;
-        e.v[1] = v;
-        e.c[1] = a[3] / a[5];
-        long bin = 1;
-        for (int k = 1; k <= m; k++) {
-            bin = bin * (m - k + 1) / k;
-            e.v[1 + k] = s_in + (size_t)(1 + k) * nd;  // a_{t-k}
-            e.c[1 + k] = (k % 2 ? 1.0 : -1.0) * (double)bin / a[5];
-        }
-        e.c[2] += a[4] / a[5];
-        launch_extrapolate(e, un, nd, stream);
-    }
+    // the initial guess, unless the previous step's corrector has already written it here
+    if (!(guess_t == t && guess_out == un)) launch_extrapolate(forward_guess(t, s_in), un, nd, stream);
+    guess_t = -1;
+    guess_out = nullptr;
     solve_A(rhs.p, un, kSolveForward);
-    // corrector, and the acceleration history of the new state: a_{t-1} .. a_{t-kFwdHist+1}
-    launch_newmark_update(u, v, ac, un, vn, an, a, nd, s_in + 3 * nd, s_out + 3 * nd, kFwdHist - 1, stream);
+    // corrector, the acceleration history of the new state (a_{t-1} .. a_{t-kFwdHist+1}) and, with
+    // next_out, the next step's initial guess from the new state, written into next_out (the same
+    // arithmetic as k_extrapolate, so the result does not depend on which kernel formed it)
+    Extrap ng{};
+    if (next_out) {
+        ng = forward_guess(t + 1, s_out);
+        guess_t = t + 1;
+        guess_out = next_out;
+    }
+    launch_newmark_update(u, v, ac, un, vn, an, a, nd, s_in + 3 * nd, s_out + 3 * nd, kFwdHist - 1, stream,
+                          next_out ? &ng : nullptr, next_out);
     stats.forward_steps++;
 }
 
@@ -286,6 +301,8 @@ void Context::forward() {
     TT_CHECK(initial_done, TT_ERR_STATE, "tt_complete_initial_state must be called first");
     given.clear();
     scratch_step = -1;
+    guess_t = -1;
+    guess_out = nullptr;
     if (reduced) {
         reduced_forward();
         forward_done = true;
@@ -297,7 +314,7 @@ void Context::forward() {
     launch_copy(slots[0].p, st_cur.p, nd3, true, true, stream);
     if (store_all) {
         for (int t = 1; t <= T; t++) {
-            newmark_step(slots[t - 1].p, slots[t].p, t);
+            newmark_step(slots[t - 1].p, slots[t].p, t, t < T ? slots[t + 1].p : nullptr);
             slot_step[t] = t;
         }
         launch_copy(slots[T].p, st_cur.p, nd3, true, true, stream);
@@ -307,7 +324,8 @@ void Context::forward() {
         int cur = 0;
         size_t k = 0;
         for (int t = 1; t <= T; t++) {
-            newmark_step(st_cur.p, st_next.p, t);
+            // (the next step writes into this step's input: its guess goes there)
+            newmark_step(st_cur.p, st_next.p, t, t < T ? st_cur.p : nullptr);
             std::swap(st_cur.p, st_next.p);
             cur = t;
             if (k < pts.size() && pts[k] == cur) {
