@@ -184,9 +184,10 @@ def step_roofline_ms(precision, iters, cheb=2):
     max(algorithmic flops / matrix peak, algorithmic bytes / HBM bandwidth), with the measured Krylov
     iterations per solve.  Kernels: the fused right-hand side (K and M contractions of two state
     combinations), the initial residual, the initial guess, the corrector; per iteration the FP64 CG
-    apply (stages z and p_old, writes p and q), the CG vector update, and the V-cycle (2 cheb operator
-    applies per smoothing level with their Chebyshev vector traffic, restriction and prolongation, the
-    dense coarsest solve on the packed inverse)."""
+    apply (stages z, reads p_old and q_old, writes p and q), the CG vector update, and the V-cycle
+    (2 cheb operator applies on the finest level, one on each intermediate level (one damped-Jacobi
+    step before the coarse-grid correction, none after), with their vector traffic, restriction and
+    prolongation, the dense coarsest solve on the packed inverse)."""
     ex, ey, ez = NEL
 
     def lvl(l):
@@ -204,7 +205,7 @@ def step_roofline_ms(precision, iters, cheb=2):
     step += t(c * nel0, 8 * (3 * nd0 + 2 * nel0) + nn0, PEAK_FP64)  # initial residual
     step += t(4 * nd0, 8 * 3 * nd0, PEAK_FP64)  # b.b, r.r
     step += t(4 * nd0, 8 * 4 * nd0, PEAK_FP64) + t(6 * nd0, 8 * 6 * nd0, PEAK_FP64)  # guess, corrector
-    it = t(c * nel0, 8 * (4 * nd0 + 2 * nel0) + nn0, PEAK_FP64)  # CG apply
+    it = t(c * nel0, 8 * (5 * nd0 + 2 * nel0) + nn0, PEAK_FP64)  # CG apply
     it += t(6 * nd0, 8 * 7 * nd0, PEAK_FP64)  # CG update
     mixed = precision == 1
     for l in range(NLVLS - 1):
@@ -212,7 +213,8 @@ def step_roofline_ms(precision, iters, cheb=2):
         es = 4 if mixed else 8
         ces = (4 if l == 0 else 2) if mixed else 8  # coefficient storage (FP32 / BF16 / FP64)
         peak = PEAK_FP32 if mixed else PEAK_FP64
-        it += 2 * cheb * t(c * nel, es * 6 * nd + ces * 2 * nel + nn, peak)
+        napp = 2 * cheb if l == 0 else 1  # (pre-smoothing steps - 1) + residual + post-smoothing steps
+        it += napp * t(c * nel, es * 6 * nd + ces * 2 * nel + nn, peak)
         _, _, ndc = lvl(l + 1)
         it += t(54 * nd, es * (nd + ndc), peak) + t(16 * nd, es * (2 * nd + ndc), peak)  # restrict, prolong
     _, _, nc = lvl(NLVLS - 1)
@@ -395,13 +397,15 @@ def main():
         "data": "synthetic (seeded filtered-noise density, half-sine line load)",
         "config": {"workload": "128x64x64 hex8 cantilever, Newmark dt=0.005, 4-level... see nlvls",
                    "nlvls": NLVLS, "rtol": 1e-8, "cheb_degree_mixed": CHEB[1], "cheb_degree_fp64": CHEB[0],
+                   "mg_coarse_pre": 1, "mg_coarse_post": 0,
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": args.steps / fp64["seconds"], "mixed_vs_fp64_speedup": fp64["seconds"] / mixed["seconds"],
         "u_final_rel_err_mixed_vs_fp64": u_err, "forward_mixed": mixed, "forward_fp64": fp64,
         "adjoint_gradient": grad, "reduced_model": reduced, "operator": ops, "roofline": roof, "cpu_baseline": cpu,
     }
     rec["config"]["workload"] = (f"128x64x64 hex8 cantilever (1,635,075 dofs), Newmark beta=1/4 dt={DT}, "
-                                 f"{rec['config']['nlvls']}-level MG, Chebyshev({CHEB[1]}) mixed / ({CHEB[0]}) FP64, "
+                                 f"{rec['config']['nlvls']}-level MG, Chebyshev({CHEB[1]}) mixed / ({CHEB[0]}) FP64 "
+                                 f"on the finest level, one damped-Jacobi step on the intermediate levels, "
                                  f"in FP64 FCG rtol 1e-8")
     print(json.dumps(rec))
 

@@ -109,7 +109,8 @@ tt_ctx* topopt_dropin_context(DM da_nodes);
    from the constructor's arguments and the options database: the reference's time grid (dt = t1/(Ts-1),
    :96-99), load (location presets 0/1, Gaussian-cosine signal; -loadsignal is not read, as in the
    reference) and outer-solver tolerances (rtol 1e-10, 200 iterations, :2750-2754, or -ksp_rtol /
-   -ksp_max_it), plus this library's own options (-precision, -slots, -cheb_degree, -filter_rtol, and the
+   -ksp_max_it), plus this library's own options (-precision, -slots, -cheb_degree, -mg_coarse_pre,
+   -mg_coarse_post, -filter_rtol, and the
    y-line half-sine pulse preset -tt_loadloc 2 -tt_loadsignal 1 -tt_pulse_steps n).  No GPU work; returns
    nonzero for a load location without a preset. */
 PetscErrorCode LinearElasticityParams(DM da_nodes, PetscScalar nu, PetscInt nlvls, PetscInt ncp, PetscInt tsc,

@@ -79,6 +79,12 @@ typedef struct tt_params {
     int coarse_direct_max;         /* largest coarsest multigrid level (dofs) solved through the dense FP64
                                       inverse; a larger one is smoothed by Chebyshev(12) on D^-1 A instead.
                                       0 = 6,476 dofs (a 320 MB dense matrix); 1 forces the Chebyshev path */
+    int mg_coarse_pre;             /* smoothing steps before the coarse-grid correction on the levels between
+                                      the finest and the coarsest (0 = 1: one damped-Jacobi step, which needs
+                                      no operator apply); the finest level smooths cheb_degree steps on
+                                      each side */
+    int mg_coarse_post;            /* smoothing steps after the coarse-grid correction on those levels
+                                      (default 0) */
 } tt_params;
 
 typedef struct tt_stats {

@@ -42,6 +42,7 @@ def test_default_params_round_trip_through_the_struct():
     assert (p.objective, p.boundaries, p.loadloc, p.loadsignal, p.pulse_steps, p.region) == (3, 1, 2, 1, 20, 2)
     assert (p.precision, p.rtol, p.maxit, p.cheb_degree) == (capi.TT_PREC_FP64, 1e-8, 200, 2)
     assert (p.rmin, p.filter_rtol, p.verbose, p.graphs) == (0.08, 1e-10, 0, 0)
+    assert (p.coarse_direct_max, p.mg_coarse_pre, p.mg_coarse_post) == (0, 1, 0)
     q = capi.default_params(nelx=8, xc=(0.0, 1.0, 0.0, 0.5, 0.0, 0.5), precision=capi.TT_PREC_MIXED)
     assert q.nelx == 8 and q.xc[3] == 0.5 and q.precision == capi.TT_PREC_MIXED and q.nely == 16
 

@@ -56,10 +56,12 @@ int main(int argc, char** argv) {
                 "\"boundaries\": %d, \"projection\": %d, \"filter\": %d, \"t1\": %.17g, \"Ts\": %d, "
                 "\"p_loadloc\": %d, \"p_loadsignal\": %d, \"p_pulse_steps\": %d, \"p_t1\": %.17g, "
                 "\"p_nsteps\": %d, \"p_rtol\": %.17g, \"p_maxit\": %d, \"p_nelx\": %d, \"p_nely\": %d, "
-                "\"p_nelz\": %d, \"p_dalpha\": %.17g, \"p_dbeta\": %.17g, \"p_objective\": %d}\n",
+                "\"p_nelz\": %d, \"p_dalpha\": %.17g, \"p_dbeta\": %.17g, \"p_objective\": %d, "
+                "\"p_cheb\": %d, \"p_coarse_pre\": %d, \"p_coarse_post\": %d}\n",
                 rc, opt->rmin, opt->V0, opt->volfrac, opt->loadloc, opt->boundaries, (int)opt->projectionFilter,
                 opt->filter, opt->t1, opt->Ts, p.loadloc, p.loadsignal, p.pulse_steps, p.t1, p.nsteps, p.rtol,
-                p.maxit, p.nelx, p.nely, p.nelz, p.dalpha, p.dbeta, p.objective);
+                p.maxit, p.nelx, p.nely, p.nelz, p.dalpha, p.dbeta, p.objective, p.cheb_degree, p.mg_coarse_pre,
+                p.mg_coarse_post);
     delete opt;
     return 0;
 }
@@ -140,6 +142,9 @@ def test_time_grid_ignores_t0(prog):
 def test_solver_options(prog):
     d = parse(prog, ["-ksp_rtol", "1e-8", "-ksp_max_it", "50"])
     assert (d["p_rtol"], d["p_maxit"]) == (1e-8, 50)
+    assert (d["p_cheb"], d["p_coarse_pre"], d["p_coarse_post"]) == (2, 1, 0)
+    d = parse(prog, ["-cheb_degree", "3", "-mg_coarse_pre", "2", "-MG_Coarse_Post", "1"])
+    assert (d["p_cheb"], d["p_coarse_pre"], d["p_coarse_post"]) == (3, 2, 1)
 
 
 def test_field_writer_layout(prog, tmp_path):

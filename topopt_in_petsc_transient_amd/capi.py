@@ -49,7 +49,7 @@ class Params(ctypes.Structure):
         ("rmin", ctypes.c_double), ("filter_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
         ("graphs", ctypes.c_int),
         ("reduction", ctypes.c_int), ("basis", ctypes.c_int), ("omega0", ctypes.c_double),
-        ("coarse_direct_max", ctypes.c_int),
+        ("coarse_direct_max", ctypes.c_int), ("mg_coarse_pre", ctypes.c_int), ("mg_coarse_post", ctypes.c_int),
     ]
 
 

@@ -633,10 +633,12 @@ PetscErrorCode LinearElasticityParams(DM da_nodes, PetscScalar nu_in, PetscInt n
     p.maxit = 200;
     PetscOptionsGetReal(NULL, NULL, "-ksp_rtol", &p.rtol, &flg);
     PetscOptionsGetInt(NULL, NULL, "-ksp_max_it", &p.maxit, &flg);
-    // options with no reference counterpart: precision, state slots, smoother degree, filter solve
+    // options with no reference counterpart: precision, state slots, smoothing steps, filter solve
     PetscOptionsGetInt(NULL, NULL, "-precision", &p.precision, &flg);
     PetscOptionsGetInt(NULL, NULL, "-slots", &p.checkpoint_slots, &flg);
     PetscOptionsGetInt(NULL, NULL, "-cheb_degree", &p.cheb_degree, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-mg_coarse_pre", &p.mg_coarse_pre, &flg);
+    PetscOptionsGetInt(NULL, NULL, "-mg_coarse_post", &p.mg_coarse_post, &flg);
     PetscOptionsGetReal(NULL, NULL, "-filter_rtol", &p.filter_rtol, &flg);
     *out = p;
     return 0;

@@ -4,7 +4,8 @@
 // scaling on the first iteration -> outer move limits -> MMA update -> design change ->
 // beta continuation -> constraint guard -> filter the new design; stop when ch <= 0.01 or
 // after maxItr iterations.  Takes the reference's command-line options (TopOpt.cc) plus
-// -precision, -slots, -ksp_rtol, -ksp_max_it, -cheb_degree, -loadsignal, -pulse_steps and
+// -precision, -slots, -ksp_rtol, -ksp_max_it, -cheb_degree, -mg_coarse_pre, -mg_coarse_post,
+// -tt_loadloc, -tt_loadsignal, -tt_pulse_steps and
 // -history <file> (one JSON line per iteration).
 #include <cstdio>
 #include <cstring>

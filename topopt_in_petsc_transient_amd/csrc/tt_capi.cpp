@@ -21,6 +21,8 @@ static void check_params(const tt_params& p) {
     TT_CHECK(p.cheb_degree >= 1 && p.cheb_degree <= 16, TT_ERR_ARG, "cheb_degree must be 1..16");
     TT_CHECK(p.rtol > 0.0 && p.maxit > 0, TT_ERR_ARG, "need rtol > 0 and maxit > 0");
     TT_CHECK(p.coarse_direct_max >= 0, TT_ERR_ARG, "coarse_direct_max must be >= 0");
+    TT_CHECK(p.mg_coarse_pre >= 0 && p.mg_coarse_pre <= 16 && p.mg_coarse_post >= 0 && p.mg_coarse_post <= 16,
+             TT_ERR_ARG, "mg_coarse_pre and mg_coarse_post must be 0..16");
     TT_CHECK(p.loadsignal != 1 || p.pulse_steps > 0, TT_ERR_ARG, "pulse_steps must be positive");
     TT_CHECK(!p.reduction || (p.basis >= 1 && p.basis <= 64), TT_ERR_ARG, "basis must be 1..64");
     TT_CHECK(!p.reduction || p.omega0 != 0.0, TT_ERR_ARG,
@@ -462,6 +464,8 @@ int tt_default_params(tt_params* p) {
     p->graphs = 0;
     p->reduction = 0; p->basis = 5; p->omega0 = 1.0;  // TopOpt.cc:157-159
     p->coarse_direct_max = 0;
+    p->mg_coarse_pre = 1;
+    p->mg_coarse_post = 0;
     return TT_OK;
 }
 

@@ -38,7 +38,8 @@ struct Multigrid {
     Context* ctx = nullptr;
     std::vector<std::unique_ptr<MGLevel>> L;
     bool mixed = false;
-    int cheb_deg = 2;
+    int cheb_deg = 2;                       // smoothing steps on each side of the finest level
+    int coarse_pre = 1, coarse_post = 0;    // before / after the correction on the intermediate levels
     double cK = 1.0, cM = 0.0;
     double cheb_lo = 0.1, cheb_hi = 1.1;
     // coarsest level

@@ -53,6 +53,8 @@ void Multigrid::build(Context* c, int nlvls, bool mixed_prec, int deg) {
     ctx = c;
     mixed = mixed_prec;
     cheb_deg = deg;
+    coarse_pre = c->prm.mg_coarse_pre > 0 ? c->prm.mg_coarse_pre : 1;
+    coarse_post = c->prm.mg_coarse_post;
     const Grid& g0 = c->g;
     const int f = 1 << (nlvls - 1);
     TT_CHECK(nlvls >= 1 && nlvls <= 8, TT_ERR_ARG, "nlvls must be in 1..8");
@@ -396,12 +398,20 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
     const long nd = lv.g.nd;
     const double lo = cheb_lo * lv.lmax, hi = cheb_hi * lv.lmax;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    // Smoothing steps: cheb_deg on each side of the finest level; coarse_pre / coarse_post (default 1 / 0)
+    // on the levels below it.  Measured on the 128x64x64 Newmark sweep (Chebyshev(2) on the finest level,
+    // 4 levels): 2 / 2 on every level 1226 mixed / 879 FP64 steps/s, 2.055 Krylov iterations per solve;
+    // 2 / 1 below the finest 1252 / 903 (2.08); 1 / 1 1299 / 943 (2.055); 1 / 0 1364 / 992 (2.06); fewer
+    // steps on the finest level cost iterations (1 / 2 there: 2.69 per solve).  The intermediate levels
+    // only need to pass the mid frequencies on: the one damped-Jacobi step and the residual are enough.
+    const int npre = l == 0 ? cheb_deg : coarse_pre;
+    const int npost = l == 0 ? cheb_deg : coarse_post;
     // pre-smoothing from x = 0 (on the finest level of a Krylov preconditioner call, b = (T) r is
     // formed by the same kernel)
     double rho0 = 1.0 / sigma;
     bool d_is_x = true;
     int k0 = 1;
-    if (cheb_deg >= 2) {
+    if (npre >= 2) {
         // x0 = D^-1 b / theta is formed inside the staging of the first step's operator apply
         const double rho1 = 1.0 / (2.0 * sigma - rho0);
         launch_cheb_first_step(lv.op, cK, cM, r64, lv.b.p, lv.dinv.p, 1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta,
@@ -416,7 +426,7 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
     } else {
         launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
     }
-    for (int k = k0; k < cheb_deg; k++) {
+    for (int k = k0; k < npre; k++) {
         const double rho1 = 1.0 / (2.0 * sigma - rho0);
         launch_cheb_step(lv.op, cK, cM, lv.b.p, lv.x.p, d_is_x ? lv.x.p : lv.d.p, lv.dinv.p, rho1 * rho0,
                          2.0 * rho1 / delta, lv.x2.p, lv.d2.p, f64, s);
@@ -436,7 +446,7 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
     launch_prolong_add(lv.g, lc.g, direct64 ? (const void*)cx.p : lc.x.p, lv.mask.p, lv.x.p, f64, direct64, s);
     // post-smoothing
     rho0 = 1.0 / sigma;
-    for (int k = 0; k < cheb_deg; k++) {
+    for (int k = 0; k < npost; k++) {
         double c0 = 0.0, c1 = 1.0 / theta;
         if (k > 0) {
             const double rho1 = 1.0 / (2.0 * sigma - rho0);
@@ -444,7 +454,7 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
             c1 = 2.0 * rho1 / delta;
             rho0 = rho1;
         }
-        if (z64 && k == cheb_deg - 1) {  // last step: z = x_new in FP64 plus the flexible-CG dots
+        if (z64 && k == npost - 1) {  // last step: z = x_new in FP64 plus the flexible-CG dots
             launch_cheb_last(lv.op, cK, cM, lv.b.p, lv.x.p, k > 0 ? lv.d.p : nullptr, lv.dinv.p, c0, c1, z64, r64,
                              rp64, first, f64, ctx->red, s);
             break;
