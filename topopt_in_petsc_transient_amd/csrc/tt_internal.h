@@ -210,6 +210,9 @@ void launch_apply_bc(const OpLevel& L, double cK, double cM, const void* u, void
 // r = b - A x
 void launch_residual(const OpLevel& L, double cK, double cM, const void* b, const void* x, void* r, bool fp64,
                      hipStream_t s);
+// x = c1 D^-1 b and r = b - A x in one pass
+void launch_resid_first(const OpLevel& L, double cK, double cM, const void* b, const void* dinv, double c1, void* x,
+                        void* r, bool fp64, hipStream_t s);
 // r = b - A x (FP64) with the partials of b.b and r.r in two partial regions
 void launch_residual_dots(const OpLevel& L, double cK, double cM, const double* b, const double* x, double* r,
                           Reducer& red, int bb_region, int rr_region, hipStream_t s);

@@ -427,6 +427,36 @@ struct EpiResid {  // r = b - A x
     TT_EPI_CALL
 };
 
+// One damped-Jacobi step from x = 0 and its residual in one pass (an intermediate multigrid level's
+// pre-smoothing): x = c1 D^-1 b is formed in staging (StageChebFirst) and written, with r = b - A x.
+template <typename T>
+struct EpiResidFirst {
+    T* x;
+    T* r;
+    struct Pre {
+        T b[3];
+    };
+    template <class S>
+    __device__ __forceinline__ void fetch(int n, const S& st, Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) p.b[c] = st.bval(3 * n + c);
+    }
+    template <class S>
+    __device__ void put(int n, uint8_t cl, const T (&yK)[3], const T (&yM)[3], const S& st, const T (&uo)[3],
+                        double (&)[2], const Pre& p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * n + c;
+            const T xi = input_at(st, cl, c, i, uo);
+            const T ax = ((cl >> c) & 1) ? xi : yK[c] + yM[c];
+            x[i] = xi;
+            r[i] = p.b[c] - ax;
+        }
+    }
+    __device__ void done(double (&)[2]) const {}
+    TT_EPI_CALL
+};
+
 // r = b - A x (FP64) with the partials of b.b and r.r, one per block into each of two regions
 struct EpiResidDots {
     const double* b;
@@ -852,6 +882,18 @@ void launch_residual(const OpLevel& L, double cK, double cM, const void* b, cons
     else
         run_apply<float, false, true>(L, cK, cM, StageOne<float>{(const float*)x},
                                       EpiResid<float>{(const float*)b, (float*)r}, s);
+}
+
+void launch_resid_first(const OpLevel& L, double cK, double cM, const void* b, const void* dinv, double c1, void* x,
+                        void* r, bool fp64, hipStream_t s) {
+    if (fp64)
+        run_apply<double, false, true>(
+            L, cK, cM, StageChebFirst<double, false>{nullptr, (const double*)b, (const double*)dinv, c1},
+            EpiResidFirst<double>{(double*)x, (double*)r}, s);
+    else
+        run_apply<float, false, true>(
+            L, cK, cM, StageChebFirst<float, false>{nullptr, (const float*)b, (const float*)dinv, (float)c1},
+            EpiResidFirst<float>{(float*)x, (float*)r}, s);
 }
 
 void launch_residual_dots(const OpLevel& L, double cK, double cM, const double* b, const double* x, double* r,

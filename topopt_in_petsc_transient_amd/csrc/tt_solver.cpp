@@ -423,9 +423,7 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
         k0 = 2;
     } else if (r64) {
         launch_cheb_first_from64(r64, lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
-    } else {
-        launch_cheb_first(lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, nd, f64, s);
-    }
+    }  // (else the single step x = D^-1 b / theta is formed by the residual's kernel below)
     for (int k = k0; k < npre; k++) {
         const double rho1 = 1.0 / (2.0 * sigma - rho0);
         launch_cheb_step(lv.op, cK, cM, lv.b.p, lv.x.p, d_is_x ? lv.x.p : lv.d.p, lv.dinv.p, rho1 * rho0,
@@ -436,7 +434,8 @@ void Multigrid::vcycle(int l, const double* r64, double* z64, const double* rp64
         rho0 = rho1;
     }
     // coarse-grid correction
-    launch_residual(lv.op, cK, cM, lv.b.p, lv.x.p, lv.r.p, f64, s);
+    if (npre == 1 && !r64) launch_resid_first(lv.op, cK, cM, lv.b.p, lv.dinv.p, 1.0 / theta, lv.x.p, lv.r.p, f64, s);
+    else launch_residual(lv.op, cK, cM, lv.b.p, lv.x.p, lv.r.p, f64, s);
     MGLevel& lc = *L[l + 1];
     // an FP32 level above the dense FP64 coarsest solve restricts into and prolongs from its FP64
     // vectors directly
