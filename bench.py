@@ -46,15 +46,20 @@ TRAFFIC_FCG64 = 84601994   # FP64 flexible-CG apply: FETCH 28416.6 KiB, WRITE 25
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
-# Chebyshev smoother degree per precision mode (each mode at its fastest setting on this workload)
+# Chebyshev smoother degree of the finest level per precision mode (each mode at its fastest setting on this
+# workload); the intermediate levels take the library's default schedule (one damped-Jacobi step before the
+# coarse-grid correction, none after) unless COARSE says otherwise
 CHEB = {0: 2, 1: 2}
+COARSE = (1, 0)
 
 
-def params(capi, precision, nsteps, slots=0, nlvls=None):
+def params(capi, precision, nsteps, slots=0, nlvls=None, coarse=None):
+    pre, post = coarse or COARSE
     return capi.default_params(
         nelx=NEL[0], nely=NEL[1], nelz=NEL[2], xc=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0), nsteps=nsteps, t1=DT * nsteps,
         pulse_steps=20, precision=precision, rtol=1e-8, maxit=200, nlvls=nlvls or NLVLS,
-        cheb_degree=CHEB.get(precision, 2), checkpoint_slots=slots, rmin=0.08, filter_rtol=1e-10)
+        cheb_degree=CHEB.get(precision, 2), checkpoint_slots=slots, rmin=0.08, filter_rtol=1e-10,
+        mg_coarse_pre=pre, mg_coarse_post=post)
 
 
 def density(capi):
@@ -116,8 +121,8 @@ def op_microbench(capi, torch, nrep):
     return out
 
 
-def forward_run(capi, x, precision, nsteps):
-    ctx = capi.Context(params(capi, precision, nsteps))
+def forward_run(capi, x, precision, nsteps, coarse=None):
+    ctx = capi.Context(params(capi, precision, nsteps, coarse=coarse))
     ctx.set_density(x)
     ctx.complete_initial_state()
     ctx.reset_stats()
@@ -312,6 +317,14 @@ def main():
     log("FP64 forward")
     fp64, u_fp64 = forward_run(capi, x, capi.TT_PREC_FP64, args.steps)
     u_err = float(np.linalg.norm(u_mixed - u_fp64) / max(np.linalg.norm(u_fp64), 1e-300))
+    # the same sweeps with Chebyshev(2) smoothing on every level (the multigrid schedule as first configured
+    # for this workload), for comparison with the default schedule above
+    every = {}
+    for name, prec in (("mixed", capi.TT_PREC_MIXED), ("fp64", capi.TT_PREC_FP64)):
+        log(f"{name} forward, Chebyshev(2) on every level")
+        r, _ = forward_run(capi, x, prec, args.steps, coarse=(2, 2))
+        every[name] = {"steps_per_s": args.steps / r["seconds"], "iters_per_solve": r["iters_per_solve"],
+                       "seconds": r["seconds"]}
     grad = None
     if not args.no_grad:
         log(f"adjoint gradient ({args.grad_steps} steps, {args.slots} slots), mixed")
@@ -397,10 +410,11 @@ def main():
         "data": "synthetic (seeded filtered-noise density, half-sine line load)",
         "config": {"workload": "128x64x64 hex8 cantilever, Newmark dt=0.005, 4-level... see nlvls",
                    "nlvls": NLVLS, "rtol": 1e-8, "cheb_degree_mixed": CHEB[1], "cheb_degree_fp64": CHEB[0],
-                   "mg_coarse_pre": 1, "mg_coarse_post": 0,
+                   "mg_coarse_pre": COARSE[0], "mg_coarse_post": COARSE[1],
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": args.steps / fp64["seconds"], "mixed_vs_fp64_speedup": fp64["seconds"] / mixed["seconds"],
         "u_final_rel_err_mixed_vs_fp64": u_err, "forward_mixed": mixed, "forward_fp64": fp64,
+        "forward_cheb2_every_level": every,
         "adjoint_gradient": grad, "reduced_model": reduced, "operator": ops, "roofline": roof, "cpu_baseline": cpu,
     }
     rec["config"]["workload"] = (f"128x64x64 hex8 cantilever (1,635,075 dofs), Newmark beta=1/4 dt={DT}, "

@@ -179,6 +179,50 @@ def test_mixed_precision_against_oracle():
     assert rel(df, df_ref) < MIXED_TOL
 
 
+_SCHED_REF = {}
+
+
+def _sched_reference():
+    """Oracle states, objective and gradient of the 32x16x16, 12-step problem (computed once)."""
+    if not _SCHED_REF:
+        po, _ = problem(nelx=32, nely=16, nelz=16, nsteps=12, nlvls=3)
+        x = random_density(O.nel(po), seed=21)
+        states = O.forward(po, x)
+        fx, df = O.objective_gradient(po, x, states)
+        _SCHED_REF.update(x=x, states=states, fx=fx, df=df)
+    return _SCHED_REF
+
+
+@pytest.mark.parametrize("pre,post", [(1, 0), (1, 1), (2, 2), (3, 1)])
+@pytest.mark.parametrize("precision", [capi.TT_PREC_FP64, capi.TT_PREC_MIXED])
+def test_intermediate_level_smoothing_schedules(precision, pre, post):
+    """Three multigrid levels, so one intermediate level: each smoothing schedule there (mg_coarse_pre steps
+    before the coarse-grid correction, mg_coarse_post after; (1, 0) is the default, a single damped-Jacobi
+    step formed together with the residual) converges to the oracle's states, objective and gradient."""
+    ref = _sched_reference()
+    _, pg = problem(nelx=32, nely=16, nelz=16, nsteps=12, nlvls=3, precision=precision,
+                    rtol=1e-12 if precision == capi.TT_PREC_FP64 else 1e-10)
+    pg.mg_coarse_pre, pg.mg_coarse_post = pre, post
+    tol = FP64_TOL if precision == capi.TT_PREC_FP64 else MIXED_TOL
+    c = _ctx(pg, ref["x"])
+    c.complete_initial_state()
+    c.forward()
+    for t in (1, 6, 12):
+        u, v, a = c.get_state(t)
+        assert rel(u, ref["states"][t, 0]) < tol and rel(a, ref["states"][t, 2]) < tol, t
+    fx, df = c.objective_gradient()
+    assert abs(fx - ref["fx"]) / abs(ref["fx"]) < tol
+    assert rel(df, ref["df"]) < tol
+    assert c.stats().max_relres <= pg.rtol * (1.0 + 1e-6)
+
+
+def test_intermediate_level_schedule_bounds_are_checked():
+    _, pg = problem(nelx=16, nely=8, nelz=8, nsteps=2, nlvls=3)
+    pg.mg_coarse_post = 17
+    with pytest.raises(capi.TTError):
+        capi.Context(pg)
+
+
 def test_pde_filter_and_projection():
     po, pg = problem()
     x = random_density(O.nel(po), seed=8, lo=0.0, hi=1.0)
