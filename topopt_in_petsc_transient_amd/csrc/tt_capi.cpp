@@ -167,7 +167,7 @@ Context::Context(const tt_params& p) : prm(p) {
 
     mg.build(this, p.nlvls, p.precision == TT_PREC_MIXED, p.cheb_degree);
 
-    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &kq2, &dinvM, &kx, &lam, &lamh[0], &lamh[1], &lamh[2], &lamh[3], &lamh[4], &lamh[5], &lamT, &mu, &nu, &mun, &nun, &hdu,
+    for (DBuf<double>* b : {&kr, &krp, &kz, &kp0, &kp1, &kq, &kq2, &dinvM, &kx, &lam, &lamh[0], &lamh[1], &lamh[2], &lamh[3], &lamh[4], &lamh[5], &lamh[6], &lamh[7], &lamT, &mu, &nu, &mun, &nun, &hdu,
                             &hddu, &hdddu, &Kl, &Ml, &rhs, &nu0})
         b->alloc(nd);
     TT_HIP(hipMemsetAsync(lamT.p, 0, nd * 8, stream));

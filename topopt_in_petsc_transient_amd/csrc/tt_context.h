@@ -156,7 +156,7 @@ struct Context {
 
     // adjoint work
     DBuf<double> lam, lamT,
-        lamh[6],  // multipliers of the later steps t+1 .. t+6 (the adjoint solves' initial guess)
+        lamh[8],  // multipliers of the later steps t+1 .. (the adjoint solves' initial guess; 7 are used)
         mu, nu, mun, nun, hdu, hddu, hdddu, Kl, Ml, rhs, df, fxs, nu0;
 
     // PDE filter

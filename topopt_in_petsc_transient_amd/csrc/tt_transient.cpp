@@ -341,10 +341,11 @@ void Context::forward() {
 }
 
 // ------------------------------------------------------------------ adjoint
-// Multipliers the adjoint solves extrapolate from: the initial guess is the degree-5 polynomial through
-// the latest 6 of them (measured on the 128x64x64 gradient: 7.4 adjoint iterations per solve with 3
-// multipliers, 5.5 with 4, 4.6 with 5, 3.7 with 6, no gain beyond).
-constexpr int kAdjHist = 6;
+// Multipliers the adjoint solves extrapolate from: the initial guess is the degree-6 polynomial through
+// the latest 7 of them.  Measured adjoint iterations per solve on the 128x64x64 gradient (200 steps,
+// rtol 1e-8): 5.4 with 4 multipliers, 4.4 with 5, 3.0 with 6, 2.5 with 7, 2.8 with 8 (a higher degree
+// amplifies the solver-tolerance noise of the stored multipliers more than it reduces the truncation).
+constexpr int kAdjHist = 7;
 static_assert(kAdjHist <= (int)(sizeof(Context::lamh) / sizeof(Context::lamh[0])) && kAdjHist <= 8, "history size");
 static_assert(Context::kFwdHist >= 1 && 2 + Context::kFwdHist <= 8, "forward guess: at most 8 extrapolated vectors");
 
