@@ -1814,7 +1814,9 @@ void launch_scale(double* x, long n, double a, hipStream_t s) {
 
 // ------------------------------------------------------------------ Newmark / adjoint vector kernels
 // corrector (LinearElasticity.cc:1363-1375): c = a[0..5].  The acceleration history of the new state
-// is shifted in the same pass: hout[0] = a, hout[j] = hin[j - 1] (j < nh), vectors n apart.
+// is shifted in the same pass: hout[0] = a, hout[j] = hin[j - 1] (j < nh), vectors n apart.  With gout,
+// the next step's initial guess sum_k gc.c[k] w_k over w = (un, vn, an, a, hin[0], hin[1]) is written as
+// well, term by term in the order k_extrapolate sums the same vectors (bit-identical results).
 struct GuessCoef {
     double c[6];
     int m;
