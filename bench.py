@@ -42,7 +42,7 @@ MFMA_EXEC_A32 = 520058 * 16384
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, FETCH scaled by 1.99922, the factor the
 # counters read for a device copy of 64 MiB; WRITE by 1.0)
 TRAFFIC_A32 = 20985242     # FP32 A apply: FETCH 7050.6 KiB, WRITE 6397.7 KiB
-TRAFFIC_FCG64 = 84601994   # FP64 flexible-CG apply: FETCH 28416.6 KiB, WRITE 25808.1 KiB
+TRAFFIC_FCG64 = 84492971   # FP64 flexible-CG apply: FETCH 28375.6 KiB, WRITE 25783.6 KiB
 NEL = (128, 64, 64)
 DT = 0.005
 NLVLS = 4      # multigrid levels of the 128x64x64 workload (coarsest 16x8x8 elements, 4,131 dofs)
