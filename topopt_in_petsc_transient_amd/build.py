@@ -17,7 +17,7 @@ ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 LIB_SOURCES = ["tt_kernels.hip", "tt_kernels2.hip", "tt_element.cpp", "tt_solver.cpp", "tt_transient.cpp", "tt_reduced.cpp",
-               "tt_capi.cpp", "tt_noalloc.hip"]
+               "tt_capi.cpp", "tt_noalloc.hip", "tt_tables.cpp"]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
              # no packed FP32 instructions (v_pk_*_f32) in any generated code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

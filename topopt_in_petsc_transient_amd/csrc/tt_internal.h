@@ -151,6 +151,12 @@ enum CoefKind { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2 };
 // `codes`.  A tile's table holds the elements that exist and touch one of its nodes: the first tile in
 // a direction has no elements on its low side, and the last tile owns fewer nodes than a full tile
 // when the grid's node count is not a multiple of the tile extent.
+// Element-local corner numbering of the reference (LinearElasticity.cc:3067-3074)
+__host__ __device__ constexpr int corner_id(int x, int y, int z) { return z * 4 + (y == 0 ? (x == 0 ? 0 : 1) : (x == 1 ? 2 : 3)); }
+__host__ __device__ constexpr int cx_of(int b) { return (b & 3) == 1 || (b & 3) == 2 ? 1 : 0; }
+__host__ __device__ constexpr int cy_of(int b) { return (b & 3) >= 2 ? 1 : 0; }
+__host__ __device__ constexpr int cz_of(int b) { return b >> 2; }
+
 struct BatchTab {
     const int4* codes;
     const int2* at;  // per position: (offset, batch count), in device memory (keeps the launch arguments small)

@@ -35,12 +35,6 @@ __device__ __forceinline__ double coef_ld(const bf16_t* p, long i) {
 
 constexpr int BS = 256;  // threads per block for all 1-D kernels
 
-// Element-local corner numbering of the reference (LinearElasticity.cc:3067-3074)
-__host__ __device__ constexpr int corner_id(int x, int y, int z) { return z * 4 + (y == 0 ? (x == 0 ? 0 : 1) : (x == 1 ? 2 : 3)); }
-__host__ __device__ constexpr int cx_of(int b) { return (b & 3) == 1 || (b & 3) == 2 ? 1 : 0; }
-__host__ __device__ constexpr int cy_of(int b) { return (b & 3) >= 2 ? 1 : 0; }
-__host__ __device__ constexpr int cz_of(int b) { return b >> 2; }
-
 // ------------------------------------------------------------------ deterministic block reduction
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
