@@ -121,8 +121,8 @@ def op_microbench(capi, torch, nrep):
     return out
 
 
-def forward_run(capi, x, precision, nsteps, coarse=None):
-    ctx = capi.Context(params(capi, precision, nsteps, coarse=coarse))
+def forward_run(capi, x, precision, nsteps, coarse=None, nlvls=None):
+    ctx = capi.Context(params(capi, precision, nsteps, nlvls=nlvls, coarse=coarse))
     ctx.set_density(x)
     ctx.complete_initial_state()
     ctx.reset_stats()
@@ -325,6 +325,15 @@ def main():
         r, _ = forward_run(capi, x, prec, args.steps, coarse=(2, 2))
         every[name] = {"steps_per_s": args.steps / r["seconds"], "iters_per_solve": r["iters_per_solve"],
                        "seconds": r["seconds"]}
+    # the same sweeps with one more multigrid level (coarsest grid 8x4x4 elements, a 675-dof dense
+    # inverse, instead of 16x8x8 and 4,131 dofs): the workload above is defined with 4 levels
+    five = {}
+    if NLVLS == 4:
+        for name, prec in (("mixed", capi.TT_PREC_MIXED), ("fp64", capi.TT_PREC_FP64)):
+            log(f"{name} forward, 5 levels")
+            r, _ = forward_run(capi, x, prec, args.steps, nlvls=5)
+            five[name] = {"steps_per_s": args.steps / r["seconds"], "iters_per_solve": r["iters_per_solve"],
+                          "seconds": r["seconds"]}
     grad = None
     if not args.no_grad:
         log(f"adjoint gradient ({args.grad_steps} steps, {args.slots} slots), mixed")
@@ -414,7 +423,7 @@ def main():
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": args.steps / fp64["seconds"], "mixed_vs_fp64_speedup": fp64["seconds"] / mixed["seconds"],
         "u_final_rel_err_mixed_vs_fp64": u_err, "forward_mixed": mixed, "forward_fp64": fp64,
-        "forward_cheb2_every_level": every,
+        "forward_cheb2_every_level": every, "forward_5_levels": five,
         "adjoint_gradient": grad, "reduced_model": reduced, "operator": ops, "roofline": roof, "cpu_baseline": cpu,
     }
     rec["config"]["workload"] = (f"128x64x64 hex8 cantilever (1,635,075 dofs), Newmark beta=1/4 dt={DT}, "

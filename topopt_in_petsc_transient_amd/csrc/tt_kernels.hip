@@ -206,21 +206,15 @@ __device__ __forceinline__ void node_contract(const T* __restrict__ sU, const T*
 // Per-node contraction of two inputs, yK = sum_e kfe * KE[a rows] u_e and yM = sum_e mfe * (M8 (x) I3)[a rows]
 // w_e, in neighbour order: the node's 27 neighbours in the outer loop, the 1-8 elements that hold both
 // the node and the neighbour in the inner one.  Each neighbour's values are read from LDS once (162
-// reads per node instead of 384 in element order); the element coefficients stay in registers.  The
-// fence per neighbour keeps the uniform matrix entries from being hoisted into scalar registers and
-// spilled.  On the 128 x 64 x 64 right-hand side this order takes 68 us against 74 us per launch.
+// reads per node instead of 384 in element order); the 8 element coefficients (element e at offset
+// (e & 1, (e >> 1) & 1, e >> 2) - 1 from the node) come in registers.  The fence per neighbour keeps
+// the uniform matrix entries from being hoisted into scalar registers and spilled.  On the 128 x 64 x 64
+// right-hand side this order takes 68 us against 74 us per launch.
 template <typename T>
 __device__ __forceinline__ void node_contract_nb(const T* __restrict__ sU, const T* __restrict__ sW,
-                                                 const T* __restrict__ sk, const T* __restrict__ sm,
+                                                 const T (&kfe)[8], const T (&mfe)[8],
                                                  const T* __restrict__ KE, const T* __restrict__ M8, int tx, int ty,
                                                  int tz, T (&yK)[3], T (&yM)[3]) {
-    T kfe[8], mfe[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int eidx = ((tz + (e >> 2)) * EYs + (ty + ((e >> 1) & 1))) * EXs + (tx + (e & 1));
-        kfe[e] = sk[eidx];
-        mfe[e] = sm[eidx];
-    }
 #pragma unroll
     for (int dz = -1; dz <= 1; dz++)
 #pragma unroll
@@ -281,24 +275,41 @@ struct StageThree {  // wK = sum_i cK_i in_i, wM = sum_i cM_i in_i (FP64)
 };
 
 struct EpiSplit;
+// The two-input form (the right-hand side) reads each node's 8 element coefficients into registers
+// instead of staging them in LDS: 28.8 instead of 35.3 KB of LDS per block, and with at most 96 VGPRs
+// (5 waves per SIMD requested) 5 instead of 4 blocks per CU.
 template <typename T, typename CT, bool TWO, bool MASKIN, class Stage, class Epi>
-__global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ KE, const T* __restrict__ M8,
-                                              const CT* __restrict__ kf, const CT* __restrict__ mf, T cK, T cM,
-                                              const uint8_t* __restrict__ mask, Stage st_in, Epi epi) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TWO ? 5 : 1))) void k_apply(
+    GridArgs g, const T* __restrict__ KE, const T* __restrict__ M8, const CT* __restrict__ kf,
+    const CT* __restrict__ mf, T cK, T cM, const uint8_t* __restrict__ mask, Stage st_in, Epi epi) {
     __shared__ T sU[HN * 3];
     __shared__ T sW[TWO ? HN * 3 : 1];
-    __shared__ T sk[EN], sm[EN];
+    __shared__ T sk[TWO ? 1 : EN], sm[TWO ? 1 : EN];
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, k0 = blockIdx.z * TZ;
     const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+    const int gi = i0 + tx, gj = j0 + ty, gk = k0 + tz;
+    const bool inside = gi < g.nx && gj < g.ny && gk < g.nz;
     Stage st = st_in;
     st.prologue();
     // Stage the node box and element coefficients.  Every global load of the
     // block is issued before the first LDS store, so the block waits for one
     // memory latency instead of one per loop trip.
     constexpr int NSV = (HN * 3 + NT - 1) / NT;
-    constexpr int NSE = (EN + NT - 1) / NT;
+    constexpr int NSE = TWO ? 1 : (EN + NT - 1) / NT;
     T val[NSV], valw[TWO ? NSV : 1];
     T ca[NSE], cb[NSE];
+    T kfe[TWO ? 8 : 1], mfe[TWO ? 8 : 1];
+    if constexpr (TWO) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int ei = gi - 1 + (e & 1), ej = gj - 1 + ((e >> 1) & 1), ek = gk - 1 + (e >> 2);
+            const bool in = inside && ei >= 0 && ei < g.ex && ej >= 0 && ej < g.ey && ek >= 0 && ek < g.ez;
+            const long ge = in ? ei + (long)g.ex * (ej + (long)g.ey * ek) : 0;
+            kfe[e] = in ? cK * (T)coef_ld(kf, ge) : (T)0;
+            mfe[e] = in ? cM * (T)coef_ld(mf, ge) : (T)0;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < NSV; q++) {
         const int v = tid + q * NT;
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
         if (TWO) valw[q] = w;
     }
 #pragma unroll
-    for (int q = 0; q < NSE; q++) {
+    for (int q = 0; q < (TWO ? 0 : NSE); q++) {
         const int e = tid + q * NT;
         const int hx = e % EXs, hy = (e / EXs) % EYs, hz = e / (EXs * EYs);
         const int gi = i0 - 1 + hx, gj = j0 - 1 + hy, gk = k0 - 1 + hz;
@@ -339,7 +350,7 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
         }
     }
 #pragma unroll
-    for (int q = 0; q < NSE; q++) {
+    for (int q = 0; q < (TWO ? 0 : NSE); q++) {
         const int e = tid + q * NT;
         if (e < EN) {
             sk[e] = ca[q];
@@ -347,15 +358,12 @@ __global__ __launch_bounds__(NT) void k_apply(GridArgs g, const T* __restrict__ 
         }
     }
     __syncthreads();
-    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
-    const int gi = i0 + tx, gj = j0 + ty, gk = k0 + tz;
-    const bool inside = gi < g.nx && gj < g.ny && gk < g.nz;
     T yK[3] = {0, 0, 0}, yM[3] = {0, 0, 0};
     long n = 0;
     uint8_t cl = 0;
     if (inside) {
         if constexpr (TWO)
-            node_contract_nb<T>(sU, sW, sk, sm, KE, M8, tx, ty, tz, yK, yM);
+            node_contract_nb<T>(sU, sW, kfe, mfe, KE, M8, tx, ty, tz, yK, yM);
         else
             node_contract<T, std::is_same<Epi, EpiSplit>::value>(sU, sk, sm, KE, M8, tx, ty, tz, yK, yM);
         n = gi + (long)g.nx * (gj + (long)g.ny * gk);
