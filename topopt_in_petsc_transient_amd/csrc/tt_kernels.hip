@@ -287,9 +287,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TWO ? 5 : 1)
     __shared__ T sk[TWO ? 1 : EN], sm[TWO ? 1 : EN];
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, k0 = blockIdx.z * TZ;
     const int tid = threadIdx.x;
+    // the thread's own node
     const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
-    const int gi = i0 + tx, gj = j0 + ty, gk = k0 + tz;
-    const bool inside = gi < g.nx && gj < g.ny && gk < g.nz;
+    const int oi = i0 + tx, oj = j0 + ty, ok = k0 + tz;
+    const bool inside = oi < g.nx && oj < g.ny && ok < g.nz;
     Stage st = st_in;
     st.prologue();
     // Stage the node box and element coefficients.  Every global load of the
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TWO ? 5 : 1)
     if constexpr (TWO) {
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const int ei = gi - 1 + (e & 1), ej = gj - 1 + ((e >> 1) & 1), ek = gk - 1 + (e >> 2);
+            const int ei = oi - 1 + (e & 1), ej = oj - 1 + ((e >> 1) & 1), ek = ok - 1 + (e >> 2);
             const bool in = inside && ei >= 0 && ei < g.ex && ej >= 0 && ej < g.ey && ek >= 0 && ek < g.ez;
             const long ge = in ? ei + (long)g.ex * (ej + (long)g.ey * ek) : 0;
             kfe[e] = in ? cK * (T)coef_ld(kf, ge) : (T)0;
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TWO ? 5 : 1)
             node_contract_nb<T>(sU, sW, kfe, mfe, KE, M8, tx, ty, tz, yK, yM);
         else
             node_contract<T, std::is_same<Epi, EpiSplit>::value>(sU, sk, sm, KE, M8, tx, ty, tz, yK, yM);
-        n = gi + (long)g.nx * (gj + (long)g.ny * gk);
+        n = oi + (long)g.nx * (oj + (long)g.ny * ok);
         cl = mask[n];
     }
     epi(inside, n, cl, yK, yM, st);
