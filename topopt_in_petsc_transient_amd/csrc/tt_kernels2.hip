@@ -97,12 +97,23 @@ struct SensDev {
 // consecutive elements, one per MFMA column, and forms per batch
 //     KE w_K and ME w_M, plus the objective's element products: KE u (objectives 3, 5), ME v (4-6),
 //     KE v (6)   (each: 2 row tiles x 6 k-steps of 24 x 24 by 24 x 16).
-// Operand layout: lane j + 16 q holds the B value of k row q, i.e. dof 4 t + q of element j in
-// k-step t, and receives D rows 4 v + q of each row tile (measured on gfx950), i.e. dofs
-// 16 R + 4 v + q.  Both are the six dofs of element j that are = q (mod 4), so every lane
-// gathers just those six values of each input vector, and lambda^T KE w_K, lambda^T ME w_M and the
+// The element's 24 dofs are taken in the order sens_dof(k): lane group q owns the two x-adjacent
+// corners of one element edge (corners 0-1, 3-2, 4-5, 7-6), whose 6 dofs are consecutive in memory
+// (3 wide loads per vector instead of 6 scattered ones), as k = 4 t + q, t = 0..5.  Operand layout:
+// lane j + 16 q holds the B value of k row q of k-step t, i.e. sens_dof(4 t + q) of element j, and
+// receives D rows 4 v + q of each row tile (measured on gfx950), i.e. k = 16 R + 4 v + q: the same
+// six dofs.  The A fragments hold KE and ME with rows and columns in the same order, so every lane
+// loads just its six values of each input vector, and lambda^T KE w_K, lambda^T ME w_M and the
 // objective's quadratic forms (u^T u, v^T v, a^T a for objectives 0-2) are sums over the four lane
 // groups (two shuffles, in a fixed order).  The element matrices stay in registers as A fragments.
+// Measured at 128x64x64 (objective 3): 75 us per launch (152 VGPRs, 3 waves per SIMD), against 82 us
+// (204 VGPRs) with the dofs = q (mod 4) in lane group q, which takes 6 scattered 8-byte loads per vector.
+__host__ __device__ constexpr int sens_corner(int q, int hi) {  // low-x / high-x corner of edge q
+    return q == 0 ? hi : q == 1 ? 3 - hi : q == 2 ? 4 + hi : 7 - hi;
+}
+__host__ __device__ constexpr int sens_dof(int k) {  // k = 4 t + q -> element dof 3 a + c
+    return 3 * sens_corner(k & 3, (k >> 2) >= 3) + (k >> 2) % 3;
+}
 typedef double sens_d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double lane_group_sum(double x) {
     x += __shfl_xor(x, 16, 64);
@@ -120,10 +131,10 @@ __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) 
     for (int R = 0; R < 2; R++)
 #pragma unroll
         for (int t = 0; t < 6; t++) {
-            const int r = 16 * R + j, c = 4 * t + q, rr = r < 24 ? r : 23;
-            const double k = s.KE[rr * 24 + c], m = s.M8[(rr / 3) * 8 + c / 3];
+            const int r = 16 * R + j, rd = sens_dof(r < 24 ? r : 23), cd = sens_dof(4 * t + q);
+            const double k = s.KE[rd * 24 + cd], m = s.M8[(rd / 3) * 8 + cd / 3];
             aK[R][t] = r < 24 ? k : 0.0;
-            aM[R][t] = (r < 24 && r % 3 == c % 3) ? m : 0.0;
+            aM[R][t] = (r < 24 && rd % 3 == cd % 3) ? m : 0.0;
         }
     const long sy = s.g.nx, sz = (long)s.g.nx * s.g.ny;
     const double hdt = 0.5 * s.dt;
@@ -135,12 +146,12 @@ __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) 
         const long ec = valid ? e : nel - 1;
         const long i = ec % s.g.ex, jj = (ec / s.g.ex) % s.g.ey, kk = ec / ((long)s.g.ex * s.g.ey);
         const long n0 = i + sy * jj + sz * kk;
+        // the low-x node of the lane group's edge: dofs sens_dof(4 t + q), t = 0..5, at x0 + t
+        const long x0 = 3 * (n0 + sy * (q & 1) + sz * (q >> 1));
         double lam[6], u[6], wk[6], wm[6], w2[6];
 #pragma unroll
         for (int t = 0; t < 6; t++) {
-            const int d = 4 * t + q, a = d / 3, c = d - 3 * a;  // element dof -> corner a, component c
-            const long node = n0 + ((a ^ (a >> 1)) & 1) + sy * ((a >> 1) & 1) + sz * (a >> 2);
-            const long x = 3 * node + c;
+            const long x = x0 + t;
             const double uu = s.u[x], x1 = s.s1[x], x2 = s.s2[x], x3 = s.s3[x];
             lam[t] = s.lam[x];
             u[t] = uu;
@@ -161,7 +172,7 @@ __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) 
                 if (MV) accVM[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aM[R][t], w2[t], accVM[R], 0, 0, 0);
                 if (KV) accVK[R] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK[R][t], w2[t], accVK[R], 0, 0, 0);
             }
-        // D row 4 v + q of row tile R is dof 4 (4 R + v) + q: the lane's value t = 4 R + v (t < 6)
+        // D row 4 v + q of row tile R is k = 4 (4 R + v) + q: the lane's value t = 4 R + v (t < 6)
         double tK = 0.0, tM = 0.0, fU = 0.0, fVM = 0.0, fVK = 0.0;
 #pragma unroll
         for (int t = 0; t < 6; t++) {
