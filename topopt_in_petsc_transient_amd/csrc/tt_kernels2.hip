@@ -106,8 +106,9 @@ struct SensDev {
 // loads just its six values of each input vector, and lambda^T KE w_K, lambda^T ME w_M and the
 // objective's quadratic forms (u^T u, v^T v, a^T a for objectives 0-2) are sums over the four lane
 // groups (two shuffles, in a fixed order).  The element matrices stay in registers as A fragments.
-// Measured at 128x64x64 (objective 3): 75 us per launch (152 VGPRs, 3 waves per SIMD), against 82 us
-// (204 VGPRs) with the dofs = q (mod 4) in lane group q, which takes 6 scattered 8-byte loads per vector.
+// Measured at 128x64x64 (objective 3): 72 us per launch (148 VGPRs, 3 waves per SIMD), against 82 us
+// (204 VGPRs) with the dofs = q (mod 4) in lane group q, which takes 6 scattered 8-byte loads per vector,
+// and 64-bit element index arithmetic (75 us with the contiguous loads alone).
 __host__ __device__ constexpr int sens_corner(int q, int hi) {  // low-x / high-x corner of edge q
     return q == 0 ? hi : q == 1 ? 3 - hi : q == 2 ? 4 + hi : 7 - hi;
 }
@@ -144,8 +145,11 @@ __global__ __launch_bounds__(256) void k_sens_mfma(SensDev s, double* partials) 
         const long e = (long)blockIdx.x * 256 + (wave * 4 + b) * 16 + j;
         const bool valid = e < nel;
         const long ec = valid ? e : nel - 1;
-        const long i = ec % s.g.ex, jj = (ec / s.g.ex) % s.g.ey, kk = ec / ((long)s.g.ex * s.g.ey);
-        const long n0 = i + sy * jj + sz * kk;
+        // element -> (i, jj, kk) in 32-bit unsigned arithmetic (two divisions; element and node
+        // indices fit in 32 bits, as everywhere in the apply kernels)
+        const unsigned eu = (unsigned)ec, exu = (unsigned)s.g.ex, eyu = (unsigned)s.g.ey;
+        const unsigned row = eu / exu, kk = row / eyu, i = eu - row * exu, jj = row - kk * eyu;
+        const long n0 = (long)i + sy * (long)jj + sz * (long)kk;
         // the low-x node of the lane group's edge: dofs sens_dof(4 t + q), t = 0..5, at x0 + t
         const long x0 = 3 * (n0 + sy * (q & 1) + sz * (q >> 1));
         double lam[6], u[6], wk[6], wm[6], w2[6];
