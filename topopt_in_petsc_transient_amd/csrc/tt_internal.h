@@ -153,6 +153,14 @@ enum CoefKind { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2 };
 // when the grid's node count is not a multiple of the tile extent.
 // Element-local corner numbering of the reference (LinearElasticity.cc:3067-3074)
 __host__ __device__ constexpr int corner_id(int x, int y, int z) { return z * 4 + (y == 0 ? (x == 0 ? 0 : 1) : (x == 1 ? 2 : 3)); }
+// (i, j, k) of node or element n on an nx x ny x * grid, x fastest: two 32-bit unsigned divisions
+// (indices fit in 32 bits; 64-bit division is a long branching sequence on the GPU)
+__host__ __device__ inline void grid_ijk(unsigned n, unsigned nx, unsigned ny, int& i, int& j, int& k) {
+    const unsigned row = n / nx, kk = row / ny;
+    i = (int)(n - row * nx);
+    j = (int)(row - kk * ny);
+    k = (int)kk;
+}
 __host__ __device__ constexpr int cx_of(int b) { return (b & 3) == 1 || (b & 3) == 2 ? 1 : 0; }
 __host__ __device__ constexpr int cy_of(int b) { return (b & 3) >= 2 ? 1 : 0; }
 __host__ __device__ constexpr int cz_of(int b) { return b >> 2; }

@@ -1192,7 +1192,8 @@ __global__ void k_restrict(GridArgs f, GridArgs c, const Tf* __restrict__ rf, co
                            Tc* __restrict__ bc) {
     const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (n >= (long)c.nx * c.ny * c.nz) return;
-    const int I = n % c.nx, J = (n / c.nx) % c.ny, K = n / ((long)c.nx * c.ny);
+    int I, J, K;
+    grid_ijk((unsigned)n, c.nx, c.ny, I, J, K);
     Tf s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
     for (int dk = -1; dk <= 1; dk++) {
@@ -1247,7 +1248,8 @@ __global__ void k_prolong_add(GridArgs f, GridArgs c, const Tc* __restrict__ xc,
                               Tf* __restrict__ xf) {
     const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (n >= (long)f.nx * f.ny * f.nz) return;
-    const int i = n % f.nx, j = (n / f.nx) % f.ny, k = n / ((long)f.nx * f.ny);
+    int i, j, k;
+    grid_ijk((unsigned)n, f.nx, f.ny, i, j, k);
     const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
     const int oi = i & 1, oj = j & 1, ok = k & 1;
     Tf s0 = 0, s1 = 0, s2 = 0;

@@ -255,7 +255,8 @@ __global__ void k_filter_apply(GA g, const double* __restrict__ KF, const double
     const long nn = (long)g.nx * g.ny * g.nz;
     double acc[1] = {0.0};
     if (n < nn) {
-        const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+        int i, j, k;
+        grid_ijk((unsigned)n, g.nx, g.ny, i, j, k);
         double s = 0.0;
 #pragma unroll
         for (int ez = 0; ez < 2; ez++)
@@ -297,7 +298,8 @@ void launch_filter_apply(const Grid& g, const double* kf, const double* u, doubl
 __global__ void k_filter_T(GA g, const double* __restrict__ x, double scale, double* __restrict__ rhs) {
     const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (n >= (long)g.nx * g.ny * g.nz) return;
-    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / ((long)g.nx * g.ny);
+    int i, j, k;
+    grid_ijk((unsigned)n, g.nx, g.ny, i, j, k);
     double s = 0.0;
     for (int ez = 0; ez < 2; ez++)
         for (int ey = 0; ey < 2; ey++)
