@@ -62,7 +62,8 @@ struct GA {
 static inline GA ga(const Grid& g) { return GA{g.nx, g.ny, g.nz, g.ex, g.ey, g.ez}; }
 
 __device__ __forceinline__ void elem_nodes(const GA& g, long e, long (&nd)[8]) {
-    const long i = e % g.ex, j = (e / g.ex) % g.ey, k = e / ((long)g.ex * g.ey);
+    int i, j, k;
+    grid_ijk((unsigned)e, g.ex, g.ey, i, j, k);
     const long n0 = i + (long)g.nx * (j + (long)g.ny * k), sy = g.nx, sz = (long)g.nx * g.ny;
     nd[0] = n0;
     nd[1] = n0 + 1;
