@@ -207,6 +207,18 @@ void Context::reduced_initial_state() {
     const double om = prm.omega0;
     const double cK0 = 1.0, cM0 = -om * om;
     mg.setup_operator(1.0, om * om);  // preconditioner: the positive-definite shifted operator
+    // The basis solves smooth with Chebyshev(cheb_deg) on every level, not with the Newmark solves'
+    // one damped-Jacobi step on the intermediate levels: flexible GMRES on the indefinite K - om^2 M
+    // needs about twice the iterations with the lighter schedule, and its Arnoldi work grows with them.
+    struct Schedule {
+        Multigrid& m;
+        const int pre, post;
+        ~Schedule() {
+            m.coarse_pre = pre;
+            m.coarse_post = post;
+        }
+    } restore{mg, mg.coarse_pre, mg.coarse_post};
+    mg.coarse_pre = mg.coarse_post = mg.cheb_deg;
     // f0 = N.F: the BC form with zero coefficients returns (1 - N).F
     double* f0 = rhs.p;
     double* f = Kl.p;
