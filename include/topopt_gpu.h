@@ -84,7 +84,9 @@ typedef struct tt_params {
                                       no operator apply); the finest level smooths cheb_degree steps on
                                       each side */
     int mg_coarse_post;            /* smoothing steps after the coarse-grid correction on those levels
-                                      (default 0) */
+                                      (default 0).  Both apply to the Newmark, adjoint and mass solves;
+                                      the reduced-order basis solves (flexible GMRES) smooth cheb_degree
+                                      steps on each side of every level */
 } tt_params;
 
 typedef struct tt_stats {
