@@ -161,6 +161,16 @@ __host__ __device__ inline void grid_ijk(unsigned n, unsigned nx, unsigned ny, i
     j = (int)(row - kk * ny);
     k = (int)kk;
 }
+// The sensitivity kernel's element dof order: k = 4 t + q (t = 0..5) is dof 3 a + c of corner
+// a = sens_corner(q, t >= 3), component c = t mod 3.  Lane group q owns the two x-adjacent corners of one
+// element edge (corners 0-1, 3-2, 4-5, 7-6), whose six dofs are consecutive in memory from the edge's
+// low-x node.
+__host__ __device__ constexpr int sens_corner(int q, int hi) {  // low-x / high-x corner of edge q
+    return q == 0 ? hi : q == 1 ? 3 - hi : q == 2 ? 4 + hi : 7 - hi;
+}
+__host__ __device__ constexpr int sens_dof(int k) {  // k = 4 t + q -> element dof 3 a + c
+    return 3 * sens_corner(k & 3, (k >> 2) >= 3) + (k >> 2) % 3;
+}
 __host__ __device__ constexpr int cx_of(int b) { return (b & 3) == 1 || (b & 3) == 2 ? 1 : 0; }
 __host__ __device__ constexpr int cy_of(int b) { return (b & 3) >= 2 ? 1 : 0; }
 __host__ __device__ constexpr int cz_of(int b) { return b >> 2; }

@@ -109,13 +109,8 @@ struct SensDev {
 // groups (two shuffles, in a fixed order).  The element matrices stay in registers as A fragments.
 // Measured at 128x64x64 (objective 3): 72 us per launch (148 VGPRs, 3 waves per SIMD), against 82 us
 // (204 VGPRs) with the dofs = q (mod 4) in lane group q, which takes 6 scattered 8-byte loads per vector,
-// and 64-bit element index arithmetic (75 us with the contiguous loads alone).
-__host__ __device__ constexpr int sens_corner(int q, int hi) {  // low-x / high-x corner of edge q
-    return q == 0 ? hi : q == 1 ? 3 - hi : q == 2 ? 4 + hi : 7 - hi;
-}
-__host__ __device__ constexpr int sens_dof(int k) {  // k = 4 t + q -> element dof 3 a + c
-    return 3 * sens_corner(k & 3, (k >> 2) >= 3) + (k >> 2) % 3;
-}
+// and 64-bit element index arithmetic (75 us with the contiguous loads alone).  sens_dof is in
+// tt_internal.h.
 typedef double sens_d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double lane_group_sum(double x) {
     x += __shfl_xor(x, 16, 64);
